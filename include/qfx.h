@@ -1,0 +1,156 @@
+/*
+ * qfx.h -- C-ABI of the MI355X (gfx950) contraction engine for qFlex's grid
+ * tensor-network path.
+ *
+ * Plain C: opaque context, plain pointers and sizes, int status codes, no
+ * exceptions and no C++/torch types across the boundary.  Device pointers
+ * handed to these calls come from qfx_alloc (the context's device arena).
+ *
+ * The reference (qFlex) performs every floating-point primitive of the path
+ * on the host through CBLAS and scalar C++ loops.  Each entry point below
+ * names the reference call site it replaces.
+ *
+ * All device work of a context is issued in order on the context's single
+ * HIP stream; calls return after enqueueing unless stated otherwise.
+ */
+#ifndef QFX_H_
+#define QFX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Interleaved complex64, identical in layout to std::complex<float>
+ * (reference s_type, src/tensor.h:30). */
+typedef struct {
+  float re, im;
+} qfx_c64;
+
+/* Interleaved complex128, identical to std::complex<double> (the amplitude
+ * type of reference src/contraction_utils.h:218-220). */
+typedef struct {
+  double re, im;
+} qfx_c128;
+
+typedef struct qfx_context qfx_context;
+
+enum qfx_status {
+  QFX_OK = 0,
+  QFX_ERR_INVALID = 1,     /* bad argument (null pointer, bad shape, ...) */
+  QFX_ERR_OUT_OF_MEMORY = 2,
+  QFX_ERR_HIP = 3,         /* HIP runtime error; see qfx_last_error() */
+  QFX_ERR_UNSUPPORTED = 4
+};
+
+/* Arithmetic forms of the complex GEMM. */
+enum qfx_precision {
+  QFX_PREC_FP32 = 0,       /* FP32 MFMA, FP32 accumulate */
+  QFX_PREC_SPLIT_BF16 = 1, /* A,B = hi+lo BF16; hi*hi + hi*lo + lo*hi, FP32 acc */
+  QFX_PREC_FP32_EXACT = 2, /* scalar fmaf in K order (defined reduction order) */
+  QFX_PREC_SPLIT_BF16X3 = 3 /* A,B = hi+mid+lo BF16; 6 cross products, FP32 acc */
+};
+
+/* Kernel classes reported by qfx_profile_read. */
+enum qfx_kernel_class {
+  QFX_K_PERMUTE = 0,
+  QFX_K_CGEMM = 1,
+  QFX_K_GEMV = 2,
+  QFX_K_DOT = 3,
+  QFX_K_ACCUMULATE = 4,
+  QFX_K_COPY = 5,
+  QFX_K_NUM_CLASSES = 6
+};
+
+/* Message of the last failed call on this host thread ("" if none). */
+const char* qfx_last_error(void);
+
+/* ---- context, stream, arena ------------------------------------------- */
+
+/* Creates a context on HIP device `device` with its own stream.  The device
+ * arena grows in chunks of at least `arena_chunk_bytes` (0 = 1 GiB).
+ * Replaces the host allocations of Tensor (src/tensor.cpp:87-155) and the
+ * scratch planning storage of ContractionData::Initialize
+ * (src/contraction_utils.cpp:179-218). */
+int qfx_context_create(int device, size_t arena_chunk_bytes, qfx_context** out);
+int qfx_context_destroy(qfx_context* ctx);
+
+/* Device memory from the arena (256-byte aligned).  Freed blocks are reused
+ * by later allocations of the same size class; memory returns to HIP only at
+ * qfx_context_destroy. */
+int qfx_alloc(qfx_context* ctx, size_t bytes, void** dptr);
+int qfx_free(qfx_context* ctx, void* dptr);
+/* Bytes currently held from HIP by the arena, and bytes handed out. */
+int qfx_arena_stats(qfx_context* ctx, size_t* reserved, size_t* in_use);
+
+/* Copies; h2d/d2h are synchronous with respect to the host buffer. */
+int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t bytes);
+int qfx_copy_d2h(qfx_context* ctx, void* dst, const void* src, size_t bytes);
+/* Replaces std::copy of Tensor::_copy / project (src/tensor.cpp:107-139,
+ * :307-315). */
+int qfx_copy_d2d(qfx_context* ctx, void* dst, const void* src, size_t bytes);
+/* Zero-fills with a kernel (graph-capture safe). */
+int qfx_zero(qfx_context* ctx, void* dst, size_t bytes);
+int qfx_synchronize(qfx_context* ctx);
+
+/* ---- the hot-path primitives ------------------------------------------ */
+
+/* Index permutation of a complex64 tensor (pure data movement, exact).
+ * src has row-major dims[0..rank-1] (first index slowest); dst receives the
+ * tensor whose axis k is src axis perm[k]:
+ *   dst[y_0..y_{r-1}] = src[x] with x_{perm[k]} = y_k.
+ * src and dst must not overlap.  Replaces Tensor::reorder's data movement
+ * (_fast_reorder/_naive_reorder, src/tensor.cpp:377-864). */
+int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* dst, int rank,
+                    const uint64_t* dims, const int* perm);
+
+/* C[m x n] = A[m x k] . B[k x n], complex64, row-major, alpha=1, beta=0.
+ * Replaces cblas_cgemm(RowMajor, NoTrans, NoTrans, m, n, k, 1, A, k, B, n,
+ * 0, C, n) in _multiply_MM (src/tensor.cpp:960-980).  k == 0 zero-fills C. */
+int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n, uint64_t k,
+                  const qfx_c64* A, const qfx_c64* B, qfx_c64* C);
+
+/* trans == 0: y[m] = A[m x k] . x[k]  (cblas_cgemv NoTrans in _multiply_Mv,
+ *             src/tensor.cpp:982-1002)
+ * trans == 1: y[k] = sum_r x[r] A[r][.], A is [m x k], x has m entries
+ *             (cblas_cgemv Trans as called by _multiply_vM,
+ *             src/tensor.cpp:1004-1025 and :1144-1151). */
+int qfx_cgemv_c64(qfx_context* ctx, int trans, uint64_t m, uint64_t k,
+                  const qfx_c64* A, const qfx_c64* x, qfx_c64* y);
+
+/* *out = sum_i x[i] y[i], unconjugated, written to device memory.
+ * Replaces cblas_cdotu_sub in _multiply_vv (src/tensor.cpp:1027-1042). */
+int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, const qfx_c64* y,
+                  qfx_c64* out);
+
+/* amps[offset + i] += (complex double) src[i], i < n, on the device.
+ * Replaces the FP64 accumulation (*amplitudes_)[output_index] += scalar of
+ * ContractionData::ContractGrid (src/contraction_utils.cpp:352), extended to
+ * a vector of terminal-cut outputs. */
+int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n, qfx_c128* amps,
+                        uint64_t offset);
+
+/* ---- timing ------------------------------------------------------------ */
+
+/* When enabled, every kernel the context launches is bracketed by a pair of
+ * HIP events on the context stream, and qfx_profile_read returns per-class
+ * launch counts, summed event time and summed algorithmic bytes and flops
+ * (permute: 16 B per element; cgemm: 8*m*n*k flop and 8*(mk+kn+mn) B). */
+int qfx_profile_enable(qfx_context* ctx, int on);
+int qfx_profile_reset(qfx_context* ctx);
+/* Synchronizes the stream, then fills the four arrays (QFX_K_NUM_CLASSES
+ * entries each; any may be NULL). */
+int qfx_profile_read(qfx_context* ctx, uint64_t* launches, double* ms, double* bytes,
+                     double* flops);
+
+/* Wall time of stream work: event at start, event at stop (synchronizes). */
+int qfx_timer_start(qfx_context* ctx);
+int qfx_timer_stop(qfx_context* ctx, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QFX_H_ */

@@ -1,0 +1,160 @@
+"""ctypes binding of libqfx.so (the C-ABI declared in include/qfx.h).
+
+Used by the tests and the benchmark to call the GPU primitives directly.
+Raises if the library is missing: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libqfx.so")
+
+FP32, SPLIT_BF16, FP32_EXACT, SPLIT_BF16X3 = 0, 1, 2, 3
+PRECISIONS = {"fp32": FP32, "split_bf16": SPLIT_BF16, "fp32_exact": FP32_EXACT,
+              "split_bf16x3": SPLIT_BF16X3}
+K_PERMUTE, K_CGEMM, K_GEMV, K_DOT, K_ACCUMULATE, K_COPY = range(6)
+KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy"]
+
+_c = ctypes
+_vp = _c.c_void_p
+_u64 = _c.c_uint64
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "qfx_last_error": (_c.c_char_p, []),
+    "qfx_context_create": (_c.c_int, [_c.c_int, _c.c_size_t, _c.POINTER(_vp)]),
+    "qfx_context_destroy": (_c.c_int, [_vp]),
+    "qfx_alloc": (_c.c_int, [_vp, _c.c_size_t, _c.POINTER(_vp)]),
+    "qfx_free": (_c.c_int, [_vp, _vp]),
+    "qfx_arena_stats": (_c.c_int, [_vp, _c.POINTER(_c.c_size_t), _c.POINTER(_c.c_size_t)]),
+    "qfx_copy_h2d": (_c.c_int, [_vp, _vp, _vp, _c.c_size_t]),
+    "qfx_copy_d2h": (_c.c_int, [_vp, _vp, _vp, _c.c_size_t]),
+    "qfx_copy_d2d": (_c.c_int, [_vp, _vp, _vp, _c.c_size_t]),
+    "qfx_zero": (_c.c_int, [_vp, _vp, _c.c_size_t]),
+    "qfx_synchronize": (_c.c_int, [_vp]),
+    "qfx_reorder_c64": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.POINTER(_u64),
+                                   _c.POINTER(_c.c_int)]),
+    "qfx_cgemm_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _u64, _vp, _vp, _vp]),
+    "qfx_cgemv_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _vp, _vp, _vp]),
+    "qfx_cdotu_c64": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
+    "qfx_accumulate_c128": (_c.c_int, [_vp, _vp, _u64, _vp, _u64]),
+    "qfx_profile_enable": (_c.c_int, [_vp, _c.c_int]),
+    "qfx_profile_reset": (_c.c_int, [_vp]),
+    "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
+                                    _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "qfx_timer_start": (_c.c_int, [_vp]),
+    "qfx_timer_stop": (_c.c_int, [_vp, _c.POINTER(_c.c_float)]),
+}
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class QfxError(RuntimeError):
+    pass
+
+
+def check(status):
+    if status != 0:
+        raise QfxError(f"qfx status {status}: {load().qfx_last_error().decode()}")
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(load().qfx_alloc(ctx.handle, max(self.nbytes, 1), ctypes.byref(p)))
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            check(load().qfx_free(self.ctx.handle, self.ptr))
+            self.ptr = None
+
+
+class Context:
+    """One qfx context (device stream + arena)."""
+
+    def __init__(self, device=0, chunk_bytes=0):
+        h = _vp()
+        check(load().qfx_context_create(device, chunk_bytes, ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            check(load().qfx_context_destroy(self.handle))
+            self.handle = None
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        buf = self.alloc(arr.nbytes)
+        check(load().qfx_copy_h2d(self.handle, buf.ptr, arr.ctypes.data, arr.nbytes))
+        return buf
+
+    def download(self, buf, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        check(load().qfx_copy_d2h(self.handle, out.ctypes.data, buf.ptr, out.nbytes))
+        return out
+
+    def sync(self):
+        check(load().qfx_synchronize(self.handle))
+
+    # primitives
+    def reorder(self, src, dst, dims, perm):
+        r = len(dims)
+        d = (ctypes.c_uint64 * max(r, 1))(*dims)
+        p = (ctypes.c_int * max(r, 1))(*perm)
+        check(load().qfx_reorder_c64(self.handle, src.ptr, dst.ptr, r, d, p))
+
+    def cgemm(self, precision, m, n, k, A, B, C):
+        check(load().qfx_cgemm_c64(self.handle, precision, m, n, k, A.ptr, B.ptr, C.ptr))
+
+    def cgemv(self, trans, m, k, A, x, y):
+        check(load().qfx_cgemv_c64(self.handle, trans, m, k, A.ptr, x.ptr, y.ptr))
+
+    def cdotu(self, n, x, y, out):
+        check(load().qfx_cdotu_c64(self.handle, n, x.ptr, y.ptr, out.ptr))
+
+    def accumulate(self, src, n, amps, offset=0):
+        check(load().qfx_accumulate_c128(self.handle, src.ptr, n, amps.ptr, offset))
+
+    def profile(self, on=True):
+        check(load().qfx_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_reset(self):
+        check(load().qfx_profile_reset(self.handle))
+
+    def profile_read(self):
+        n = 6
+        la = (ctypes.c_uint64 * n)()
+        ms = (ctypes.c_double * n)()
+        by = (ctypes.c_double * n)()
+        fl = (ctypes.c_double * n)()
+        check(load().qfx_profile_read(self.handle, la, ms, by, fl))
+        return {KERNEL_CLASSES[i]: {"launches": la[i], "ms": ms[i], "bytes": by[i],
+                                    "flops": fl[i]} for i in range(n)}
+
+    def timer_start(self):
+        check(load().qfx_timer_start(self.handle))
+
+    def timer_stop(self):
+        t = ctypes.c_float()
+        check(load().qfx_timer_stop(self.handle, ctypes.byref(t)))
+        return t.value

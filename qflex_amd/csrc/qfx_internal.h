@@ -1,0 +1,70 @@
+// Internal declarations shared by the qfx HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "qfx.h"
+
+struct QfxProfRec {
+  int cls;
+  hipEvent_t start, stop;
+  double bytes, flops;
+};
+
+struct qfx_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // Arena: chunks from hipMalloc, bump-allocated; freed blocks are kept in
+  // per-size free lists (sizes rounded to the arena's size classes).
+  struct Chunk {
+    char* base;
+    size_t size, used;
+  };
+  std::vector<Chunk> chunks;
+  size_t chunk_bytes = size_t(1) << 30;
+  std::map<size_t, std::vector<void*>> free_lists;
+  std::map<void*, size_t> live;  // pointer -> rounded size
+  size_t in_use = 0;
+  // Device workspace for split reductions (grows on demand).
+  void* work = nullptr;
+  size_t work_bytes = 0;
+  // Profiling.
+  bool profiling = false;
+  std::vector<QfxProfRec> pending;
+  std::vector<hipEvent_t> event_pool;
+  uint64_t launches[QFX_K_NUM_CLASSES] = {};
+  double ms[QFX_K_NUM_CLASSES] = {};
+  double bytes[QFX_K_NUM_CLASSES] = {};
+  double flops[QFX_K_NUM_CLASSES] = {};
+  hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+};
+
+// Error plumbing: every entry point returns a status and records a message.
+int qfx_set_error(int status, const std::string& msg);
+int qfx_hip_error(hipError_t e, const char* what);
+
+#define QFX_HIP(call)                                      \
+  do {                                                     \
+    hipError_t e_ = (call);                                \
+    if (e_ != hipSuccess) return qfx_hip_error(e_, #call); \
+  } while (0)
+
+#define QFX_CHECK_CTX(ctx) \
+  if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context")
+
+// Bracket a launch with profiling events (no-op unless profiling is on).
+struct QfxLaunchScope {
+  qfx_context* ctx;
+  QfxProfRec rec;
+  bool active;
+  QfxLaunchScope(qfx_context* c, int cls, double bytes, double flops);
+  ~QfxLaunchScope();
+};
+
+// Device workspace of at least `bytes` (reused across calls on the stream).
+int qfx_workspace(qfx_context* ctx, size_t bytes, void** out);

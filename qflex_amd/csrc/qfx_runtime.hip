@@ -1,0 +1,372 @@
+// Context, device arena, copies, accumulation and timing of the qfx C-ABI.
+#include <cstring>
+#include <mutex>
+
+#include "qfx_internal.h"
+
+static thread_local std::string g_last_error;
+
+int qfx_set_error(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+int qfx_hip_error(hipError_t e, const char* what) {
+  return qfx_set_error(QFX_ERR_HIP, std::string("HIP error ") + hipGetErrorName(e) + " (" +
+                                        hipGetErrorString(e) + ") in " + what);
+}
+
+extern "C" const char* qfx_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------- context
+
+extern "C" int qfx_context_create(int device, size_t arena_chunk_bytes, qfx_context** out) {
+  if (!out) return qfx_set_error(QFX_ERR_INVALID, "null output pointer");
+  int n = 0;
+  QFX_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return qfx_set_error(QFX_ERR_INVALID, "device " + std::to_string(device) +
+                                              " out of range (devices: " + std::to_string(n) + ")");
+  QFX_HIP(hipSetDevice(device));
+  auto* ctx = new qfx_context();
+  ctx->device = device;
+  if (arena_chunk_bytes) ctx->chunk_bytes = arena_chunk_bytes;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return qfx_hip_error(e, "hipStreamCreateWithFlags");
+  }
+  QFX_HIP(hipEventCreate(&ctx->timer_start));
+  QFX_HIP(hipEventCreate(&ctx->timer_stop));
+  *out = ctx;
+  return QFX_OK;
+}
+
+extern "C" int qfx_context_destroy(qfx_context* ctx) {
+  QFX_CHECK_CTX(ctx);
+  hipSetDevice(ctx->device);
+  hipStreamSynchronize(ctx->stream);
+  for (auto& r : ctx->pending) {
+    hipEventDestroy(r.start);
+    hipEventDestroy(r.stop);
+  }
+  for (auto ev : ctx->event_pool) hipEventDestroy(ev);
+  for (auto& c : ctx->chunks) hipFree(c.base);
+  if (ctx->timer_start) hipEventDestroy(ctx->timer_start);
+  if (ctx->timer_stop) hipEventDestroy(ctx->timer_stop);
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return QFX_OK;
+}
+
+// ---------------------------------------------------------------- arena
+
+// Size classes: 256-byte granules below 64 KiB, otherwise quarter-octave
+// steps (<= 19% rounding waste).
+static size_t round_class(size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  if (bytes <= 65536) return (bytes + 255) & ~size_t(255);
+  size_t p = size_t(1) << (63 - __builtin_clzll(bytes));
+  size_t step = p / 4;
+  return (bytes + step - 1) / step * step;
+}
+
+extern "C" int qfx_alloc(qfx_context* ctx, size_t bytes, void** dptr) {
+  QFX_CHECK_CTX(ctx);
+  if (!dptr) return qfx_set_error(QFX_ERR_INVALID, "null output pointer");
+  size_t sz = round_class(bytes);
+  auto it = ctx->free_lists.find(sz);
+  if (it != ctx->free_lists.end() && !it->second.empty()) {
+    *dptr = it->second.back();
+    it->second.pop_back();
+  } else {
+    void* p = nullptr;
+    for (auto& c : ctx->chunks) {
+      if (c.size - c.used >= sz) {
+        p = c.base + c.used;
+        c.used += sz;
+        break;
+      }
+    }
+    if (!p) {
+      size_t csz = std::max(ctx->chunk_bytes, sz);
+      char* base = nullptr;
+      hipError_t e = hipMalloc(&base, csz);
+      if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return qfx_set_error(QFX_ERR_OUT_OF_MEMORY,
+                             "device arena: cannot reserve " + std::to_string(csz) +
+                                 " bytes (" + hipGetErrorString(e) + ")");
+      }
+      ctx->chunks.push_back({base, csz, sz});
+      p = base;
+    }
+    *dptr = p;
+  }
+  ctx->live[*dptr] = sz;
+  ctx->in_use += sz;
+  return QFX_OK;
+}
+
+extern "C" int qfx_free(qfx_context* ctx, void* dptr) {
+  QFX_CHECK_CTX(ctx);
+  if (!dptr) return QFX_OK;
+  auto it = ctx->live.find(dptr);
+  if (it == ctx->live.end())
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_free: pointer not from this arena");
+  ctx->free_lists[it->second].push_back(dptr);
+  ctx->in_use -= it->second;
+  ctx->live.erase(it);
+  return QFX_OK;
+}
+
+extern "C" int qfx_arena_stats(qfx_context* ctx, size_t* reserved, size_t* in_use) {
+  QFX_CHECK_CTX(ctx);
+  size_t r = 0;
+  for (auto& c : ctx->chunks) r += c.size;
+  if (reserved) *reserved = r;
+  if (in_use) *in_use = ctx->in_use;
+  return QFX_OK;
+}
+
+int qfx_workspace(qfx_context* ctx, size_t bytes, void** out) {
+  if (ctx->work_bytes < bytes) {
+    if (ctx->work) {
+      // The stream may still read the old workspace; it is not freed to HIP,
+      // only returned to the arena after the stream drains.
+      QFX_HIP(hipStreamSynchronize(ctx->stream));
+      qfx_free(ctx, ctx->work);
+      ctx->work = nullptr;
+    }
+    size_t sz = std::max<size_t>(bytes, 1 << 20);
+    int st = qfx_alloc(ctx, sz, &ctx->work);
+    if (st != QFX_OK) return st;
+    ctx->work_bytes = sz;
+  }
+  *out = ctx->work;
+  return QFX_OK;
+}
+
+// ---------------------------------------------------------------- profiling
+
+static hipEvent_t pool_event(qfx_context* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+QfxLaunchScope::QfxLaunchScope(qfx_context* c, int cls, double bytes, double flops)
+    : ctx(c), active(c->profiling) {
+  if (!active) return;
+  rec.cls = cls;
+  rec.bytes = bytes;
+  rec.flops = flops;
+  rec.start = pool_event(ctx);
+  rec.stop = pool_event(ctx);
+  hipEventRecord(rec.start, ctx->stream);
+}
+
+QfxLaunchScope::~QfxLaunchScope() {
+  if (!active) return;
+  hipEventRecord(rec.stop, ctx->stream);
+  ctx->pending.push_back(rec);
+  // Bound the number of outstanding event pairs.
+  if (ctx->pending.size() >= 4096) {
+    hipEventSynchronize(ctx->pending.back().stop);
+    for (auto& r : ctx->pending) {
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, r.start, r.stop);
+      ctx->launches[r.cls] += 1;
+      ctx->ms[r.cls] += ms;
+      ctx->bytes[r.cls] += r.bytes;
+      ctx->flops[r.cls] += r.flops;
+      ctx->event_pool.push_back(r.start);
+      ctx->event_pool.push_back(r.stop);
+    }
+    ctx->pending.clear();
+  }
+}
+
+extern "C" int qfx_profile_enable(qfx_context* ctx, int on) {
+  QFX_CHECK_CTX(ctx);
+  ctx->profiling = on != 0;
+  return QFX_OK;
+}
+
+extern "C" int qfx_profile_reset(qfx_context* ctx) {
+  QFX_CHECK_CTX(ctx);
+  QFX_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->pending) {
+    ctx->event_pool.push_back(r.start);
+    ctx->event_pool.push_back(r.stop);
+  }
+  ctx->pending.clear();
+  for (int i = 0; i < QFX_K_NUM_CLASSES; ++i) {
+    ctx->launches[i] = 0;
+    ctx->ms[i] = ctx->bytes[i] = ctx->flops[i] = 0;
+  }
+  return QFX_OK;
+}
+
+extern "C" int qfx_profile_read(qfx_context* ctx, uint64_t* launches, double* ms,
+                                double* bytes, double* flops) {
+  QFX_CHECK_CTX(ctx);
+  QFX_HIP(hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->pending) {
+    float t = 0.f;
+    QFX_HIP(hipEventElapsedTime(&t, r.start, r.stop));
+    ctx->launches[r.cls] += 1;
+    ctx->ms[r.cls] += t;
+    ctx->bytes[r.cls] += r.bytes;
+    ctx->flops[r.cls] += r.flops;
+    ctx->event_pool.push_back(r.start);
+    ctx->event_pool.push_back(r.stop);
+  }
+  ctx->pending.clear();
+  for (int i = 0; i < QFX_K_NUM_CLASSES; ++i) {
+    if (launches) launches[i] = ctx->launches[i];
+    if (ms) ms[i] = ctx->ms[i];
+    if (bytes) bytes[i] = ctx->bytes[i];
+    if (flops) flops[i] = ctx->flops[i];
+  }
+  return QFX_OK;
+}
+
+extern "C" int qfx_timer_start(qfx_context* ctx) {
+  QFX_CHECK_CTX(ctx);
+  QFX_HIP(hipEventRecord(ctx->timer_start, ctx->stream));
+  return QFX_OK;
+}
+
+extern "C" int qfx_timer_stop(qfx_context* ctx, float* ms) {
+  QFX_CHECK_CTX(ctx);
+  QFX_HIP(hipEventRecord(ctx->timer_stop, ctx->stream));
+  QFX_HIP(hipEventSynchronize(ctx->timer_stop));
+  float t = 0.f;
+  QFX_HIP(hipEventElapsedTime(&t, ctx->timer_start, ctx->timer_stop));
+  if (ms) *ms = t;
+  return QFX_OK;
+}
+
+// ---------------------------------------------------------------- copies
+
+extern "C" int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
+  QFX_CHECK_CTX(ctx);
+  if (!bytes) return QFX_OK;
+  if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_h2d: null pointer");
+  QFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  QFX_HIP(hipStreamSynchronize(ctx->stream));
+  return QFX_OK;
+}
+
+extern "C" int qfx_copy_d2h(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
+  QFX_CHECK_CTX(ctx);
+  if (!bytes) return QFX_OK;
+  if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_d2h: null pointer");
+  QFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  QFX_HIP(hipStreamSynchronize(ctx->stream));
+  return QFX_OK;
+}
+
+__global__ void qfx_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                uint64_t n16) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+__global__ void qfx_copy_bytes_kernel(const unsigned char* __restrict__ src,
+                                      unsigned char* __restrict__ dst, uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+static unsigned grid_for(uint64_t n, unsigned block) {
+  uint64_t g = (n + block - 1) / block;
+  if (g > 65536) g = 65536;
+  if (g == 0) g = 1;
+  return unsigned(g);
+}
+
+extern "C" int qfx_copy_d2d(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
+  QFX_CHECK_CTX(ctx);
+  if (!bytes || dst == src) return QFX_OK;
+  if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_d2d: null pointer");
+  QfxLaunchScope scope(ctx, QFX_K_COPY, 2.0 * bytes, 0);
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) % 16 == 0) {
+    uint64_t n16 = bytes / 16;
+    qfx_copy_kernel<<<grid_for(n16, 256), 256, 0, ctx->stream>>>(
+        static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16);
+  } else {
+    qfx_copy_bytes_kernel<<<grid_for(bytes, 256), 256, 0, ctx->stream>>>(
+        static_cast<const unsigned char*>(src), static_cast<unsigned char*>(dst), bytes);
+  }
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
+__global__ void qfx_zero_kernel(unsigned char* __restrict__ dst, uint64_t n) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) dst[i] = 0;
+}
+
+__global__ void qfx_zero16_kernel(uint4* __restrict__ dst, uint64_t n16) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; i < n16; i += stride) dst[i] = make_uint4(0, 0, 0, 0);
+}
+
+extern "C" int qfx_zero(qfx_context* ctx, void* dst, size_t bytes) {
+  QFX_CHECK_CTX(ctx);
+  if (!bytes) return QFX_OK;
+  if (!dst) return qfx_set_error(QFX_ERR_INVALID, "qfx_zero: null pointer");
+  QfxLaunchScope scope(ctx, QFX_K_COPY, double(bytes), 0);
+  if ((reinterpret_cast<uintptr_t>(dst) | bytes) % 16 == 0)
+    qfx_zero16_kernel<<<grid_for(bytes / 16, 256), 256, 0, ctx->stream>>>(
+        static_cast<uint4*>(dst), bytes / 16);
+  else
+    qfx_zero_kernel<<<grid_for(bytes, 256), 256, 0, ctx->stream>>>(
+        static_cast<unsigned char*>(dst), bytes);
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
+extern "C" int qfx_synchronize(qfx_context* ctx) {
+  QFX_CHECK_CTX(ctx);
+  QFX_HIP(hipStreamSynchronize(ctx->stream));
+  return QFX_OK;
+}
+
+// ---------------------------------------------------------------- accumulate
+
+__global__ void qfx_accumulate_kernel(const float2* __restrict__ src, uint64_t n,
+                                      double2* __restrict__ amps) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    float2 v = src[i];
+    double2 a = amps[i];
+    a.x += double(v.x);
+    a.y += double(v.y);
+    amps[i] = a;
+  }
+}
+
+extern "C" int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n,
+                                   qfx_c128* amps, uint64_t offset) {
+  QFX_CHECK_CTX(ctx);
+  if (!n) return QFX_OK;
+  if (!src || !amps) return qfx_set_error(QFX_ERR_INVALID, "qfx_accumulate_c128: null pointer");
+  QfxLaunchScope scope(ctx, QFX_K_ACCUMULATE, 8.0 * n + 32.0 * n, 2.0 * n);
+  unsigned g = unsigned((n + 255) / 256);
+  qfx_accumulate_kernel<<<g, 256, 0, ctx->stream>>>(reinterpret_cast<const float2*>(src), n,
+                                                     reinterpret_cast<double2*>(amps) + offset);
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}

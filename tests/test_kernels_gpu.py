@@ -1,0 +1,189 @@
+"""GPU parity of the C-ABI primitives against the oracle's numpy semantics.
+
+Reorders are pure data movement and must be bit-exact.  Complex GEMMs are
+compared norm-wise against complex128 (numpy) with the tolerance written per
+precision; the defined-order FP32_EXACT path is also compared bit-exactly on
+integer-valued inputs, where every partial sum is exact.
+"""
+import numpy as np
+import pytest
+
+from qflex_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def rand_c64(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def relerr(x, ref):
+    return float(np.linalg.norm((x - ref).ravel()) / np.linalg.norm(ref.ravel()))
+
+
+REORDER_CASES = [
+    # (dims, perm)
+    ([2, 2, 2], [1, 2, 0]),                       # reference known answer
+    ([2] * 10, [9, 4, 6, 2, 5, 1, 7, 8, 3, 0]),   # reference RightTen (tensor_test.cpp:271)
+    ([2] * 12, [11, 10, 9, 8, 7, 6, 5, 4, 3, 2, 1, 0]),
+    ([16, 16, 16, 16], [3, 0, 2, 1]),
+    ([16, 16, 16, 16, 16], [4, 3, 2, 1, 0]),
+    ([16, 16, 16, 16, 16, 2], [5, 0, 1, 2, 3, 4]),
+    ([2, 16, 16, 16, 16, 16, 16], [1, 2, 0, 3, 4, 5, 6]),
+    ([16] * 6, [0, 1, 2, 3, 5, 4]),               # inner swap, outer kept
+    ([16] * 6, [1, 0, 2, 3, 4, 5]),               # outer swap, inner kept
+    ([256, 4096], [1, 0]),                        # plain transpose
+    ([4, 1, 8, 1], [2, 3, 0, 1]),                 # unit axes
+    ([3, 5, 6], [2, 0, 1]),                       # non powers of two
+    ([7], [0]),
+    ([2, 3, 2, 5], [3, 1, 0, 2]),
+]
+
+
+@pytest.mark.parametrize("dims,perm", REORDER_CASES)
+def test_reorder_bit_exact(ctx, dims, perm):
+    rng = np.random.default_rng(len(dims) * 31 + sum(dims))
+    x = rand_c64(rng, dims)
+    ref = np.ascontiguousarray(x.transpose(perm))
+    src = ctx.upload(x)
+    dst = ctx.alloc(x.nbytes)
+    ctx.reorder(src, dst, dims, perm)
+    got = ctx.download(dst, np.complex64, ref.shape)
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    src.free()
+    dst.free()
+
+
+def test_reorder_random_binary_permutations(ctx):
+    rng = np.random.default_rng(7)
+    for nbits in (11, 16, 20, 22):
+        x = rand_c64(rng, [2] * nbits)
+        src = ctx.upload(x)
+        dst = ctx.alloc(x.nbytes)
+        for _ in range(3):
+            perm = [int(p) for p in rng.permutation(nbits)]
+            ctx.reorder(src, dst, [2] * nbits, perm)
+            got = ctx.download(dst, np.complex64, x.shape)
+            assert np.array_equal(got, np.ascontiguousarray(x.transpose(perm))), (nbits, perm)
+        src.free()
+        dst.free()
+
+
+def test_reorder_large_rank28_like(ctx):
+    # 2^24 complex64 (128 MiB): a 6-axis bond tensor permutation.
+    rng = np.random.default_rng(3)
+    dims = [16, 16, 16, 16, 16, 16]
+    x = rand_c64(rng, dims)
+    perm = [2, 5, 0, 4, 1, 3]
+    src = ctx.upload(x)
+    dst = ctx.alloc(x.nbytes)
+    ctx.reorder(src, dst, dims, perm)
+    got = ctx.download(dst, np.complex64, [dims[p] for p in perm])
+    assert np.array_equal(got, np.ascontiguousarray(x.transpose(perm)))
+    src.free()
+    dst.free()
+
+
+GEMM_SHAPES = [
+    (2, 2, 2), (4, 2, 2), (16, 16, 16), (64, 16, 8), (100, 50, 37), (128, 64, 16),
+    (256, 256, 256), (4096, 256, 256), (1024, 512, 256), (513, 129, 65), (64, 4096, 16),
+    (1, 64, 4096), (1, 3, 5), (300, 1, 77), (5, 1, 3),
+]
+TOL = {capi.FP32: 2e-6, capi.FP32_EXACT: 2e-6, capi.SPLIT_BF16X3: 5e-6, capi.SPLIT_BF16: 3e-4}
+
+
+@pytest.mark.parametrize("m,n,k", GEMM_SHAPES)
+@pytest.mark.parametrize("prec", [capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16, capi.SPLIT_BF16X3])
+def test_cgemm_vs_complex128(ctx, m, n, k, prec):
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    a = rand_c64(rng, (m, k))
+    b = rand_c64(rng, (k, n))
+    ref = a.astype(np.complex128) @ b.astype(np.complex128)
+    A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+    ctx.cgemm(prec, m, n, k, A, B, C)
+    got = ctx.download(C, np.complex64, (m, n))
+    err = relerr(got.astype(np.complex128), ref)
+    assert err <= TOL[prec], (m, n, k, prec, err)
+    for buf in (A, B, C):
+        buf.free()
+
+
+def test_cgemm_reference_known_answer(ctx):
+    # tests/src/tensor_test.cpp:496-530: A(a,b,c)=0..7, B(b,c,d)=8..1 -> {20,14,100,78}
+    a = np.arange(8, dtype=np.complex64).reshape(2, 4)
+    b = np.arange(8, 0, -1).astype(np.complex64).reshape(4, 2)
+    for prec in (capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16):
+        A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(32)
+        ctx.cgemm(prec, 2, 2, 4, A, B, C)
+        got = ctx.download(C, np.complex64, (2, 2))
+        assert np.array_equal(got.ravel(), np.array([20, 14, 100, 78], dtype=np.complex64))
+
+
+def test_cgemm_exact_integer_inputs_bit_exact(ctx):
+    rng = np.random.default_rng(11)
+    for (m, n, k) in [(64, 16, 8), (200, 96, 40), (130, 70, 300)]:
+        a = (rng.integers(-8, 8, (m, k)) + 1j * rng.integers(-8, 8, (m, k))).astype(np.complex64)
+        b = (rng.integers(-8, 8, (k, n)) + 1j * rng.integers(-8, 8, (k, n))).astype(np.complex64)
+        ref = (a.astype(np.complex128) @ b.astype(np.complex128)).astype(np.complex64)
+        for prec in (capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16):
+            A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+            ctx.cgemm(prec, m, n, k, A, B, C)
+            got = ctx.download(C, np.complex64, (m, n))
+            assert np.array_equal(got, ref), (m, n, k, prec)
+
+
+@pytest.mark.parametrize("m,k", [(1, 1), (7, 300), (4096, 256), (3, 65536)])
+def test_cgemv(ctx, m, k):
+    rng = np.random.default_rng(m + k)
+    a = rand_c64(rng, (m, k))
+    x = rand_c64(rng, (k,))
+    xt = rand_c64(rng, (m,))
+    A, X, Y = ctx.upload(a), ctx.upload(x), ctx.alloc(m * 8)
+    ctx.cgemv(0, m, k, A, X, Y)
+    got = ctx.download(Y, np.complex64, (m,))
+    assert relerr(got.astype(np.complex128), a.astype(np.complex128) @ x) < 2e-6
+    XT, YT = ctx.upload(xt), ctx.alloc(k * 8)
+    ctx.cgemv(1, m, k, A, XT, YT)
+    got = ctx.download(YT, np.complex64, (k,))
+    assert relerr(got.astype(np.complex128), xt.astype(np.complex128) @ a) < 2e-6
+
+
+@pytest.mark.parametrize("n", [1, 5, 256, 65537, 1 << 22])
+def test_cdotu_unconjugated(ctx, n):
+    rng = np.random.default_rng(n)
+    x, y = rand_c64(rng, (n,)), rand_c64(rng, (n,))
+    X, Y, O = ctx.upload(x), ctx.upload(y), ctx.alloc(8)
+    ctx.cdotu(n, X, Y, O)
+    got = complex(ctx.download(O, np.complex64, (1,))[0])
+    ref = complex(np.sum(x.astype(np.complex128) * y.astype(np.complex128)))
+    assert abs(got - ref) <= 2e-6 * np.sqrt(n) * max(1.0, abs(ref))
+
+
+def test_accumulate_fp64(ctx):
+    rng = np.random.default_rng(5)
+    amps = np.zeros(80, dtype=np.complex128)
+    amps[:] = rng.standard_normal(80)
+    D = ctx.upload(amps)
+    total = amps.copy()
+    for s in range(3):
+        v = rand_c64(rng, (64,))
+        V = ctx.upload(v)
+        ctx.accumulate(V, 64, D, offset=16)
+        total[16:] += v.astype(np.complex128)
+    got = ctx.download(D, np.complex128, (80,))
+    assert np.array_equal(got, total)
+
+
+def test_profile_counts_launches(ctx):
+    rng = np.random.default_rng(1)
+    a = rand_c64(rng, (256, 256))
+    A, B, C = ctx.upload(a), ctx.upload(a), ctx.alloc(a.nbytes)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.cgemm(capi.FP32, 256, 256, 256, A, B, C)
+    ctx.reorder(A, C, [256, 256], [1, 0])
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["cgemm"]["launches"] == 1 and prof["permute"]["launches"] == 1
+    assert prof["cgemm"]["flops"] == 8.0 * 256 ** 3
+    assert prof["cgemm"]["ms"] > 0
