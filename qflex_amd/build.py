@@ -32,7 +32,8 @@ HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-result", "-I" + INCLUDE]
 CXX = os.environ.get("CXX", "g++")
-CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + INCLUDE, "-I" + HOST]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-fmacro-prefix-map=" + ROOT + "/=",
+             "-I" + INCLUDE, "-I" + HOST]
 
 
 def _run(cmd):

@@ -1,0 +1,636 @@
+// Device-resident grid contraction interpreter (see contraction_utils.h).
+// Parser, validation and planning follow reference
+// src/contraction_utils.cpp:16-221 and :358-610; the interpreter keeps the
+// semantics of :223-354.
+#include "contraction_utils.h"
+
+#include <algorithm>
+#include <iostream>
+#include <regex>
+#include <sstream>
+#include <unordered_set>
+
+#include "device.h"
+#include "errors.h"
+
+namespace qflex {
+
+namespace {
+
+ContractionStats g_last_stats;
+
+std::size_t product(const std::vector<std::size_t>& d) {
+  std::size_t p = 1;
+  for (auto x : d) p *= x;
+  return p;
+}
+
+}  // namespace
+
+const ContractionStats& last_contraction_stats() { return g_last_stats; }
+
+std::size_t ContractionData::View::size() const { return product(dims); }
+
+// ------------------------------------------------------------ naming
+
+std::string index_name(const std::vector<std::size_t>& p1, const std::vector<std::size_t>& p2) {
+  if (p1.size() == 2 && p2.size() == 2)
+    return utils::concat("(", p1[0], ",", p1[1], "),(", p2[0], ",", p2[1], ")");
+  if (p1.size() == 3 && p2.size() == 3)
+    return utils::concat("(", p1[0], ",", p1[1], ",", p1[2], "),(", p2[0], ",", p2[1], ",", p2[2],
+                         ")");
+  if (p1.size() == 2 && p2.empty()) return utils::concat("(", p1[0], ",", p1[1], "),(o)");
+  std::stringstream ss;
+  ss << "Failed to construct tensor name with the following vectors: p1 = [";
+  for (const auto& p : p1) ss << p << ',';
+  ss << "] and p2 = [";
+  for (const auto& p : p2) ss << p << ',';
+  ss << "].";
+  throw ERROR_MSG(ss.str());
+}
+
+std::string index_name(const std::vector<std::vector<std::size_t>>& tensors) {
+  if (tensors.size() == 2) {
+    try {
+      return index_name(tensors.at(0), tensors.at(1));
+    } catch (const std::string& err_msg) {
+      throw ERROR_MSG("Failed to call index_name(). Error:\n\t[", err_msg, "]");
+    }
+  }
+  if (tensors.size() == 1) {
+    try {
+      return index_name(tensors.at(0), {});
+    } catch (const std::string& err_msg) {
+      throw ERROR_MSG("Failed to call index_name(). Error:\n\t[", err_msg, "]");
+    }
+  }
+  throw ERROR_MSG("Failed to construct tensor name with input tensors size: ", tensors.size(),
+                  ".");
+}
+
+std::vector<std::size_t> get_qubit_coords(std::size_t q, std::size_t J) {
+  std::size_t i = q / J;
+  return {i, q - i * J};
+}
+
+bool find_grid_coord_in_list(const std::optional<std::vector<std::vector<std::size_t>>>& coord_list,
+                             const std::size_t i, const std::size_t j) {
+  return coord_list.has_value() &&
+         std::find(coord_list.value().begin(), coord_list.value().end(),
+                   std::vector<std::size_t>({i, j})) != coord_list.value().end();
+}
+
+// ------------------------------------------------------------ parsing
+
+void ordering_data_to_contraction_ordering(const QflexInput& input,
+                                           std::list<ContractionOperation>* ordering) {
+  if (ordering == nullptr) throw ERROR_MSG("Ordering must be non-null.");
+  static const std::regex cut_value_regex("\\([0-9,]*\\)");
+  std::string operation;
+  for (const auto& line : input.ordering.instructions) {
+    auto error_msg = [&line](const auto& msg) {
+      return ERROR_MSG("Parsing failed on line: '", line, "' with error: ", msg);
+    };
+    auto check_index = [&input, &error_msg](long int index) {
+      const std::size_t grid_size = input.grid.I * input.grid.J;
+      if (index < 0 || std::size_t(index) >= grid_size)
+        throw error_msg(utils::concat("Index ", index, " must be within grid boundaries."));
+    };
+    if (line.empty() || line[0] == '#') continue;
+    std::stringstream ss(line);
+    ss >> operation;
+    if (operation == "expand") {
+      std::string patch;
+      long int index = -1;
+      ss >> patch >> index;
+      check_index(index);
+      auto position = get_qubit_coords(index, input.grid.J);
+      if (find_grid_coord_in_list(input.grid.qubits_off, position[0], position[1]))
+        throw error_msg("Index must specify an active qubit.");
+      ordering->emplace_back(ExpandPatch(patch, position));
+    } else if (operation == "cut") {
+      std::vector<std::size_t> values;
+      std::string values_str;
+      ss >> values_str;
+      if (!std::regex_match(values_str, cut_value_regex))
+        throw error_msg("Cut values must be comma-separated ints, e.g. (0,1,3).");
+      values_str = values_str.substr(1, values_str.size() - 2);
+      if (!values_str.empty()) {
+        std::stringstream vs(values_str);
+        std::string v;
+        while (std::getline(vs, v, ',')) values.push_back(v.empty() ? 0 : std::stoul(v));
+      }
+      long int index_1 = -1, index_2 = -1;
+      ss >> index_1;
+      check_index(index_1);
+      auto position_1 = get_qubit_coords(index_1, input.grid.J);
+      if (find_grid_coord_in_list(input.grid.qubits_off, position_1[0], position_1[1]))
+        throw error_msg("Index 1 must specify an active qubit.");
+      if (ss.eof()) {
+        ordering->emplace_back(CutIndex({position_1}, values));
+      } else {
+        ss >> index_2;
+        check_index(index_2);
+        auto position_2 = get_qubit_coords(index_2, input.grid.J);
+        if (find_grid_coord_in_list(input.grid.qubits_off, position_2[0], position_2[1]))
+          throw error_msg("Index 2 must specify an active qubit.");
+        if (index_1 < index_2)
+          ordering->emplace_back(CutIndex({position_1, position_2}, values));
+        else
+          ordering->emplace_back(CutIndex({position_2, position_1}, values));
+      }
+    } else if (operation == "merge") {
+      std::string patch_1, patch_2;
+      ss >> patch_1 >> patch_2;
+      ordering->emplace_back(MergePatches(patch_1, patch_2));
+    } else {
+      throw error_msg("Received an invalid operation in config.");
+    }
+  }
+  ValidateOrdering(*ordering);
+}
+
+// ------------------------------------------------------------ validation
+
+void ValidateOrdering(const std::list<ContractionOperation>& ordering) {
+  struct PatchState {
+    bool is_active = false;  // expanded, no cut since
+    bool is_used = false;    // expanded, and a cut happened since
+    bool is_merged = false;  // merged into another patch
+  };
+  std::unordered_map<std::string, PatchState> patches;
+  std::unordered_set<std::string> cut_indices, used_tensors;
+  std::string error_msg;
+  for (const auto& op : ordering) {
+    switch (op.op_type) {
+      case ContractionOperation::EXPAND: {
+        auto& st = patches[op.expand.id];
+        if (st.is_used || st.is_merged)
+          error_msg = utils::concat(error_msg, "\nTensor at (", op.expand.tensor[0], ",",
+                                    op.expand.tensor[1], ") is added to non-empty patch ",
+                                    op.expand.id, " after a cut.");
+        std::string name = utils::concat("(", op.expand.tensor[0], ",", op.expand.tensor[1], ")");
+        if (!used_tensors.insert(name).second)
+          error_msg = utils::concat(error_msg, "\nTensor ", name, " is contracted multiple times.");
+        st.is_active = true;
+        break;
+      }
+      case ContractionOperation::CUT: {
+        for (auto& p : patches)
+          if (p.second.is_active) p.second.is_used = true;
+        std::string index;
+        try {
+          index = index_name(op.cut.tensors);
+        } catch (const std::string& err_msg) {
+          throw ERROR_MSG("Failed to during CUT. Error:\n\t[", err_msg, "]");
+        }
+        if (!cut_indices.insert(index).second)
+          error_msg = utils::concat(error_msg, "\nIndex ", index, " is cut multiple times.");
+        break;
+      }
+      case ContractionOperation::MERGE: {
+        if (patches[op.merge.source_id].is_merged)
+          error_msg = utils::concat(error_msg, "\nPatch ", op.merge.source_id,
+                                    " is merged multiple times.");
+        if (patches[op.merge.target_id].is_used)
+          error_msg = utils::concat(error_msg, "\nPatch ", op.merge.source_id,
+                                    " is merged into non-empty patch ", op.merge.target_id,
+                                    " after a cut.");
+        patches[op.merge.source_id].is_merged = true;
+        patches[op.merge.target_id].is_active = true;
+        break;
+      }
+    }
+  }
+  if (!error_msg.empty()) throw error_msg;
+}
+
+// ------------------------------------------------------------ planning
+
+ContractionData ContractionData::Initialize(const std::list<ContractionOperation>& ordering,
+                                            std::vector<std::vector<Tensor>>* tensor_grid,
+                                            std::vector<std::complex<double>>* amplitudes) {
+  if (tensor_grid == nullptr) throw ERROR_MSG("Tensor grid must be non-null.");
+  if (amplitudes == nullptr) throw ERROR_MSG("Amplitude return vector must be non-null.");
+  ContractionData data;
+  data.tensor_grid_ = tensor_grid;
+  data.amplitudes_ = amplitudes;
+
+  std::unordered_map<std::string, std::size_t> index_dimension;
+  std::vector<std::vector<std::vector<std::string>>> grid_indices(tensor_grid->size());
+  std::size_t max_grid = 0;
+  for (std::size_t i = 0; i < tensor_grid->size(); ++i) {
+    grid_indices[i].resize((*tensor_grid)[i].size());
+    for (std::size_t j = 0; j < (*tensor_grid)[i].size(); ++j) {
+      const Tensor& t = (*tensor_grid)[i][j];
+      grid_indices[i][j] = t.get_indices();
+      for (const auto& index : t.get_indices())
+        index_dimension[index] = t.get_index_to_dimension().at(index);
+      if (t.data()) max_grid = std::max(max_grid, t.size());
+    }
+  }
+
+  // Symbolic pass.  Bond cuts remove their index; terminal cuts keep the
+  // output index open with one entry per requested value.
+  std::unordered_map<std::string, std::vector<std::string>> patch_indices;
+  std::unordered_map<std::string, std::size_t> side_buffers;
+  auto sym_diff = [](const std::vector<std::string>& a, const std::vector<std::string>& b) {
+    return _vector_subtraction(_vector_union(a, b), _vector_intersection(a, b));
+  };
+  auto size_of = [&](const std::vector<std::string>& idx) {
+    std::size_t s = 1;
+    for (const auto& x : idx) s *= index_dimension.at(x);
+    return s;
+  };
+  for (const auto& op : ordering) {
+    switch (op.op_type) {
+      case ContractionOperation::EXPAND: {
+        auto& pi = patch_indices[op.expand.id];
+        pi = sym_diff(pi, grid_indices.at(op.expand.tensor[0]).at(op.expand.tensor[1]));
+        auto& mx = data.patch_max_size_[op.expand.id];
+        mx = std::max(mx, size_of(pi));
+        break;
+      }
+      case ContractionOperation::CUT: {
+        try {
+          const std::string cut_index = index_name(op.cut.tensors);
+          int side = 0;
+          for (const auto& tensor : op.cut.tensors) {
+            auto& indices = grid_indices.at(tensor[0]).at(tensor[1]);
+            const std::size_t full = (*tensor_grid)[tensor[0]][tensor[1]].size();
+            const std::size_t dim = index_dimension.count(cut_index) ? index_dimension[cut_index] : 1;
+            if (op.cut.tensors.size() > 1) {
+              indices = _vector_subtraction(indices, {cut_index});
+              side_buffers[cut_copy_name(op.cut.tensors, side)] = full / std::max<std::size_t>(dim, 1);
+            } else {
+              const std::size_t count = op.cut.values.empty() ? dim : op.cut.values.size();
+              index_dimension[cut_index] = count;
+              side_buffers[cut_copy_name(op.cut.tensors, side)] =
+                  full / std::max<std::size_t>(dim, 1) * count;
+            }
+            ++side;
+          }
+        } catch (const std::string& err_msg) {
+          throw ERROR_MSG("Failed during CUT. Error:\n\t[", err_msg, "]");
+        }
+        break;
+      }
+      case ContractionOperation::MERGE: {
+        auto& pi = patch_indices[op.merge.target_id];
+        pi = sym_diff(pi, patch_indices[op.merge.source_id]);
+        auto& mx = data.patch_max_size_[op.merge.target_id];
+        mx = std::max(mx, size_of(pi));
+        break;
+      }
+    }
+  }
+
+  std::size_t max_size = max_grid;
+  std::vector<std::size_t> unique_sizes;
+  for (const auto& p : data.patch_max_size_) {
+    max_size = std::max(max_size, p.second);
+    if (std::find(unique_sizes.begin(), unique_sizes.end(), p.second) == unique_sizes.end())
+      unique_sizes.push_back(p.second);
+  }
+  std::sort(unique_sizes.begin(), unique_sizes.end());
+  // Two general buffers (out-of-place reorders of both operands), one
+  // result buffer per size class, one buffer per patch, the cut / gather
+  // buffers and the FP64 amplitude vector.
+  std::size_t allocated = 2 * max_size;
+  for (auto s : unique_sizes) allocated += s;
+  for (const auto& p : data.patch_max_size_) allocated += p.second;
+  for (const auto& p : side_buffers) allocated += p.second;
+  allocated += 2 * amplitudes->size();
+  const double alloc_size = double(allocated) * sizeof(s_type);
+  if (alloc_size > double(global::memory_limit)) {
+    throw ERROR_MSG("Required space (", utils::readable_memory_string(alloc_size),
+                    ") exceeds memory limit (",
+                    utils::readable_memory_string(double(global::memory_limit)),
+                    "). Cancelling simulation.");
+  }
+  if (global::verbose > 0)
+    std::cerr << WARN_MSG("Allocating ", utils::readable_memory_string(alloc_size),
+                          " for this simulation.")
+              << std::endl;
+  data.stats_.planned_bytes = std::size_t(alloc_size);
+
+  data.scratch_.reserve(3 + unique_sizes.size() + data.patch_max_size_.size() +
+                        side_buffers.size());
+  auto add = [&](const std::string& name, std::size_t elements) {
+    try {
+      data.scratch_.push_back(Tensor({""}, {std::max<std::size_t>(elements, 1)}));
+    } catch (const std::string& err_msg) {
+      throw ERROR_MSG("Failed to call Tensor(). Error:\n\t[", err_msg, "]");
+    }
+    data.scratch_map_[name] = data.scratch_.size() - 1;
+  };
+  add(kGeneralSpace, max_size);
+  add(kGeneralSpaceB, max_size);
+  for (auto s : unique_sizes) add(result_space(s), s);
+  for (const auto& p : data.patch_max_size_) add(p.first, p.second);
+  for (const auto& p : side_buffers) add(p.first, p.second);
+  add(kAmplitudeSpace, 2 * amplitudes->size());
+  return data;
+}
+
+// ------------------------------------------------------------ execution
+
+ContractionData::View ContractionData::contract_pair(const View& A, const View& B, s_type* out) {
+  std::vector<std::string> left = _vector_subtraction(A.indices, B.indices);
+  std::vector<std::string> right = _vector_subtraction(B.indices, A.indices);
+  std::vector<std::string> common = _vector_intersection(A.indices, B.indices);
+  std::unordered_map<std::string, std::size_t> adim, bdim;
+  for (std::size_t i = 0; i < A.indices.size(); ++i) adim[A.indices[i]] = A.dims[i];
+  for (std::size_t i = 0; i < B.indices.size(); ++i) bdim[B.indices[i]] = B.dims[i];
+  std::size_t left_dim = 1, right_dim = 1, common_dim = 1;
+  for (const auto& x : left) left_dim *= adim[x];
+  for (const auto& x : right) right_dim *= bdim[x];
+  for (std::size_t i = 0; i < common.size(); ++i) {
+    if (adim[common[i]] != bdim[common[i]])
+      throw ERROR_MSG("Common indices must have matching dimensions, but at index: ", i,
+                      ", the dimensions are A: ", adim[common[i]], ", B: ", bdim[common[i]], ".");
+    common_dim *= adim[common[i]];
+  }
+  const s_type* pa = A.ptr;
+  const s_type* pb = B.ptr;
+  auto a_order = _vector_union(left, common);
+  if (a_order != A.indices) {
+    s_type* buf = get_scratch(kGeneralSpace).data();
+    device_reorder(A.ptr, buf, A.indices, A.dims, a_order);
+    pa = buf;
+    stats_.reorders++;
+    stats_.reorder_elements += double(A.size());
+  }
+  auto b_order = _vector_union(common, right);
+  if (b_order != B.indices) {
+    s_type* buf = get_scratch(kGeneralSpaceB).data();
+    device_reorder(B.ptr, buf, B.indices, B.dims, b_order);
+    pb = buf;
+    stats_.reorders++;
+    stats_.reorder_elements += double(B.size());
+  }
+  device_multiply(pa, pb, out, left_dim, right_dim, common_dim, !left.empty(), !right.empty());
+  stats_.multiplies++;
+  stats_.flops += 8.0 * double(left_dim) * double(right_dim) * double(common_dim);
+  View C;
+  C.ptr = out;
+  C.indices = _vector_union(left, right);
+  for (const auto& x : left) C.dims.push_back(adim[x]);
+  for (const auto& x : right) C.dims.push_back(bdim[x]);
+  return C;
+}
+
+ContractionData::View ContractionData::project_view(const View& v, const std::string& index,
+                                                    std::size_t value,
+                                                    const std::vector<std::size_t>* gather_values,
+                                                    s_type* buffer) {
+  auto it = std::find(v.indices.begin(), v.indices.end(), index);
+  if (it == v.indices.end())
+    throw ERROR_MSG("Index: '", index, "' has to be equal to indices[0]: '",
+                    v.indices.empty() ? std::string() : v.indices[0], "'.");
+  const std::size_t p = std::size_t(it - v.indices.begin());
+  const std::size_t dim = v.dims[p];
+  std::size_t pre = 1, post = 1;
+  for (std::size_t a = 0; a < p; ++a) pre *= v.dims[a];
+  for (std::size_t a = p + 1; a < v.dims.size(); ++a) post *= v.dims[a];
+  std::vector<std::size_t> picks;
+  if (gather_values)
+    picks = *gather_values;
+  else
+    picks = {value};
+  for (auto x : picks)
+    if (x >= dim)
+      throw ERROR_MSG("index_value: ", x, " must be contained in [0, ", dim, ").");
+  View out;
+  out.indices = v.indices;
+  out.dims = v.dims;
+  if (!gather_values) {
+    out.indices.erase(out.indices.begin() + p);
+    out.dims.erase(out.dims.begin() + p);
+  } else {
+    out.dims[p] = picks.size();
+  }
+  if (pre == 1 && picks.size() == 1) {
+    out.ptr = v.ptr + picks[0] * post;  // contiguous slice: zero copy
+    return out;
+  }
+  if (buffer == nullptr) throw ERROR_MSG("No buffer for projection of index ", index, ".");
+  // out[a][k][rest] = v[a][picks[k]][rest]
+  for (std::size_t a = 0; a < pre; ++a)
+    for (std::size_t k = 0; k < picks.size(); ++k)
+      qfx_check(qfx_copy_d2d(device(), buffer + (a * picks.size() + k) * post,
+                             v.ptr + (a * dim + picks[k]) * post, post * sizeof(s_type)),
+                "qfx_copy_d2d");
+  out.ptr = buffer;
+  return out;
+}
+
+void ContractionData::emit_output(const View& output, std::size_t output_index,
+                                  const std::vector<OpenOutput>& open_outputs) {
+  std::vector<std::string> open_names;
+  std::size_t P = 1;
+  for (const auto& o : open_outputs) {
+    open_names.push_back(o.index);
+    P *= o.count;
+  }
+  if (!_vector_subtraction(output.indices, open_names).empty() || output.size() != P) {
+    std::string desc = "Tensor of rank " + std::to_string(output.indices.size()) + ": ";
+    for (std::size_t i = 0; i < output.indices.size(); ++i)
+      desc += output.indices[i] + " -> " + std::to_string(output.dims[i]) +
+              (i + 1 < output.indices.size() ? ", " : "");
+    throw ERROR_MSG("Contraction did not complete; final tensor is: ", desc);
+  }
+  const std::size_t offset = output_index * P;
+  if (offset + P > amplitudes_->size())
+    throw ERROR_MSG("Output index ", offset + P - 1, " is outside the amplitude vector of size ",
+                    amplitudes_->size(), ".");
+  const s_type* src = output.ptr;
+  if (open_names != output.indices) {
+    s_type* buf = get_scratch(kGeneralSpaceB).data();
+    if (buf == output.ptr) buf = get_scratch(kGeneralSpace).data();
+    device_reorder(output.ptr, buf, output.indices, output.dims, open_names);
+    src = buf;
+  }
+  qfx_check(qfx_accumulate_c128(device(), reinterpret_cast<const qfx_c64*>(src), P,
+                                reinterpret_cast<qfx_c128*>(get_scratch(kAmplitudeSpace).data()),
+                                offset),
+            "qfx_accumulate_c128");
+  stats_.slices++;
+}
+
+void ContractionData::run(std::list<ContractionOperation>& ordering, std::size_t output_index,
+                          std::unordered_map<std::string, bool>& active_patches,
+                          std::vector<OpenOutput> open_outputs) {
+  std::vector<std::pair<std::vector<std::size_t>, View>> restore;
+  bool have_output = false;
+  View output;
+  while (!ordering.empty()) {
+    const ContractionOperation op = ordering.front();
+    ordering.pop_front();
+    switch (op.op_type) {
+      case ContractionOperation::EXPAND: {
+        const View& next = grid_views_.at(op.expand.tensor[0]).at(op.expand.tensor[1]);
+        if (!active_patches[op.expand.id]) {
+          patches_[op.expand.id] = next;
+          active_patches[op.expand.id] = true;
+          continue;
+        }
+        const std::string rid = result_space(patch_max_size_[op.expand.id]);
+        View res;
+        try {
+          res = contract_pair(patches_[op.expand.id], next, get_scratch(rid).data());
+        } catch (const std::string& err_msg) {
+          throw ERROR_MSG("Failed to call multiply(). Error:\n\t[", err_msg, "]");
+        }
+        if (ordering.empty()) {
+          output = res;
+          have_output = true;
+          continue;
+        }
+        std::swap(scratch_map_[op.expand.id], scratch_map_[rid]);
+        patches_[op.expand.id] = res;
+        continue;
+      }
+      case ContractionOperation::CUT: {
+        const std::string index = index_name(op.cut.tensors);
+        const auto& pa = op.cut.tensors[0];
+        View& va = grid_views_.at(pa[0]).at(pa[1]);
+        const View saved_a = va;
+        std::size_t dim = 0;
+        for (std::size_t a = 0; a < saved_a.indices.size(); ++a)
+          if (saved_a.indices[a] == index) dim = saved_a.dims[a];
+        std::vector<std::size_t> values = op.cut.values;
+        if (values.empty())
+          for (std::size_t v = 0; v < dim; ++v) values.push_back(v);
+        if (op.cut.tensors.size() > 1) {
+          const auto& pb = op.cut.tensors[1];
+          View& vb = grid_views_.at(pb[0]).at(pb[1]);
+          const View saved_b = vb;
+          s_type* buf_a = get_scratch(cut_copy_name(op.cut.tensors, 0)).data();
+          s_type* buf_b = get_scratch(cut_copy_name(op.cut.tensors, 1)).data();
+          for (std::size_t val : values) {
+            try {
+              grid_views_[pa[0]][pa[1]] = project_view(saved_a, index, val, nullptr, buf_a);
+              grid_views_[pb[0]][pb[1]] = project_view(saved_b, index, val, nullptr, buf_b);
+            } catch (const std::string& err_msg) {
+              throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+            }
+            std::list<ContractionOperation> rest(ordering);
+            auto active = active_patches;
+            run(rest, output_index, active, open_outputs);
+          }
+          grid_views_[pa[0]][pa[1]] = saved_a;
+          grid_views_[pb[0]][pb[1]] = saved_b;
+        } else {
+          // Terminal cut: keep the output index open (one entry per value).
+          bool full = values.size() == dim;
+          for (std::size_t v = 0; full && v < values.size(); ++v) full = values[v] == v;
+          if (!full) {
+            try {
+              grid_views_[pa[0]][pa[1]] = project_view(
+                  saved_a, index, 0, &values, get_scratch(cut_copy_name(op.cut.tensors, 0)).data());
+            } catch (const std::string& err_msg) {
+              throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+            }
+            restore.push_back({pa, saved_a});
+          }
+          open_outputs.push_back({index, values.size()});
+          continue;
+        }
+        for (auto it = restore.rbegin(); it != restore.rend(); ++it)
+          grid_views_[it->first[0]][it->first[1]] = it->second;
+        return;  // the rest of the ordering ran inside the cut loop
+      }
+      case ContractionOperation::MERGE: {
+        if (!active_patches[op.merge.target_id]) {
+          patches_[op.merge.target_id] = patches_[op.merge.source_id];
+          active_patches[op.merge.target_id] = true;
+          continue;
+        }
+        const std::string rid = result_space(patch_max_size_[op.merge.target_id]);
+        View res;
+        try {
+          res = contract_pair(patches_[op.merge.source_id], patches_[op.merge.target_id],
+                              get_scratch(rid).data());
+        } catch (const std::string& err_msg) {
+          throw ERROR_MSG("Failed to call multiply(). Error:\n\t[", err_msg, "]");
+        }
+        if (ordering.empty()) {
+          output = res;
+          have_output = true;
+          continue;
+        }
+        std::swap(scratch_map_[op.merge.target_id], scratch_map_[rid]);
+        patches_[op.merge.target_id] = res;
+        continue;
+      }
+    }
+  }
+  if (!have_output) throw ERROR_MSG("Contraction did not complete; no final tensor was produced.");
+  emit_output(output, output_index, open_outputs);
+  for (auto it = restore.rbegin(); it != restore.rend(); ++it)
+    grid_views_[it->first[0]][it->first[1]] = it->second;
+}
+
+void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
+                                   std::size_t output_index,
+                                   std::unordered_map<std::string, bool> active_patches) {
+  const bool top = depth_ == 0;
+  if (top) {
+    grid_views_.assign(tensor_grid_->size(), {});
+    for (std::size_t i = 0; i < tensor_grid_->size(); ++i) {
+      grid_views_[i].resize((*tensor_grid_)[i].size());
+      for (std::size_t j = 0; j < (*tensor_grid_)[i].size(); ++j) {
+        const Tensor& t = (*tensor_grid_)[i][j];
+        grid_views_[i][j].ptr = t.data();
+        grid_views_[i][j].indices = t.get_indices();
+        grid_views_[i][j].dims = t.get_dimensions();
+      }
+    }
+    patches_.clear();
+    ContractionStats fresh;
+    fresh.planned_bytes = stats_.planned_bytes;
+    stats_ = fresh;
+    Tensor& amps = get_scratch(kAmplitudeSpace);
+    qfx_check(qfx_zero(device(), amps.data(), amps.capacity() * sizeof(s_type)), "qfx_zero");
+  }
+  ++depth_;
+  try {
+    run(ordering, output_index, active_patches, {});
+  } catch (...) {
+    --depth_;
+    throw;
+  }
+  --depth_;
+  if (top) {
+    std::vector<std::complex<double>> device_amps(amplitudes_->size());
+    if (!device_amps.empty())
+      qfx_check(qfx_copy_d2h(device(), device_amps.data(), get_scratch(kAmplitudeSpace).data(),
+                             device_amps.size() * sizeof(std::complex<double>)),
+                "qfx_copy_d2h");
+    for (std::size_t i = 0; i < device_amps.size(); ++i) (*amplitudes_)[i] += device_amps[i];
+    g_last_stats = stats_;
+  }
+}
+
+void ContractGrid(const std::list<ContractionOperation>& ordering,
+                  std::vector<std::vector<Tensor>>* tensor_grid,
+                  std::vector<std::complex<double>>* amplitudes) {
+  if (tensor_grid == nullptr) throw ERROR_MSG("Tensor grid must be non-null.");
+  if (amplitudes == nullptr) throw ERROR_MSG("Amplitude return vector must be non-null.");
+  ContractionData data;
+  try {
+    data = ContractionData::Initialize(ordering, tensor_grid, amplitudes);
+  } catch (const std::string& err_msg) {
+    throw ERROR_MSG("Failed to call Initialize(). Error:\n\t[", err_msg, "]");
+  }
+  std::unordered_map<std::string, bool> active_patches;
+  for (const auto& patch : data.scratch_list()) active_patches[patch] = false;
+  try {
+    data.ContractGrid(ordering, /*output_index=*/0, active_patches);
+  } catch (const std::string& err_msg) {
+    throw ERROR_MSG("Failed to call ContractGrid(). Error:\n\t[", err_msg, "]");
+  }
+}
+
+}  // namespace qflex

@@ -1,0 +1,167 @@
+// Grid contraction: operation IR, ordering parser, validation and the
+// device-resident contraction interpreter.
+//
+// Interface of the reference's src/contraction_utils.h:21-220.  The
+// interpreter keeps the reference's semantics (expand / cut / merge, cut
+// recursion, FP64 amplitude accumulation, output_index arithmetic) but runs
+// on the GPU:
+//   * tensors never leave the device arena; grid tensors and projections
+//     are zero-copy views, and operands are reordered out of place;
+//   * a normal cut iterates its values and sums into the same amplitudes;
+//   * a terminal cut (one tensor: an output qubit) keeps its output index
+//     open instead of re-running the rest of the ordering once per value,
+//     so one pass yields the whole vector of outputs of that region; the
+//     vector is scattered to the amplitudes in the reference's
+//     output_index order;
+//   * amplitudes are summed in FP64 on the device across slices.
+#ifndef QFLEX_AMD_CONTRACTION_UTILS_H_
+#define QFLEX_AMD_CONTRACTION_UTILS_H_
+
+#include <complex>
+#include <list>
+#include <optional>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "global.h"
+#include "input.h"
+#include "tensor.h"
+#include "utils.h"
+
+namespace qflex {
+
+std::string index_name(const std::vector<std::size_t>& p1, const std::vector<std::size_t>& p2);
+std::string index_name(const std::vector<std::vector<std::size_t>>& tensors);
+std::vector<std::size_t> get_qubit_coords(std::size_t q, std::size_t J);
+bool find_grid_coord_in_list(const std::optional<std::vector<std::vector<std::size_t>>>& coord_list,
+                             const std::size_t i, const std::size_t j);
+
+struct ExpandPatch {
+  ExpandPatch() {}
+  ExpandPatch(std::string id, std::vector<std::size_t> tensor) : id(id), tensor(tensor) {}
+  std::string id;
+  std::vector<std::size_t> tensor;
+};
+
+struct CutIndex {
+  CutIndex() {}
+  CutIndex(std::vector<std::vector<std::size_t>> tensors, std::vector<std::size_t> values = {})
+      : tensors(tensors), values(values) {}
+  // Two tensors: a bond cut; one tensor: a terminal cut (output value).
+  std::vector<std::vector<std::size_t>> tensors;
+  // Values to assign; empty means all values of the index.
+  std::vector<std::size_t> values;
+};
+
+struct MergePatches {
+  MergePatches() {}
+  MergePatches(std::string source_id, std::string target_id)
+      : source_id(source_id), target_id(target_id) {}
+  std::string source_id;
+  std::string target_id;
+};
+
+struct ContractionOperation {
+ public:
+  ContractionOperation(ExpandPatch expand) : op_type(EXPAND), expand(expand) {}
+  ContractionOperation(CutIndex cut) : op_type(CUT), cut(cut) {}
+  ContractionOperation(MergePatches merge) : op_type(MERGE), merge(merge) {}
+
+  enum OpType { EXPAND, CUT, MERGE };
+  OpType op_type;
+  const ExpandPatch expand;
+  const CutIndex cut;
+  const MergePatches merge;
+};
+
+void ordering_data_to_contraction_ordering(const QflexInput& input,
+                                           std::list<ContractionOperation>* ordering);
+
+// Work counters of the last ContractGrid (algorithmic, per the textbook
+// definitions: cgemm 8*m*n*k flop; a reorder moves 16 bytes per element).
+struct ContractionStats {
+  std::size_t slices = 0;        // executions of the innermost suffix
+  std::size_t multiplies = 0;
+  double flops = 0;
+  double reorder_elements = 0;
+  std::size_t reorders = 0;
+  std::size_t planned_bytes = 0;
+};
+
+class ContractionData {
+ public:
+  static ContractionData Initialize(const std::list<ContractionOperation>& ordering,
+                                    std::vector<std::vector<Tensor>>* tensor_grid,
+                                    std::vector<std::complex<double>>* amplitudes);
+
+  static constexpr char kGeneralSpace[] = "_general_internal_";
+  static constexpr char kGeneralSpaceB[] = "_general_internal_b_";
+  static constexpr char kResultSpace[] = "_result_internal_";
+  static constexpr char kAmplitudeSpace[] = "_amplitudes_internal_";
+
+  void ContractGrid(std::list<ContractionOperation> ordering, std::size_t output_index,
+                    std::unordered_map<std::string, bool> active_patches);
+
+  Tensor& get_scratch(std::string id) { return scratch_[scratch_map_.at(id)]; }
+
+  static std::string cut_copy_name(std::vector<std::vector<std::size_t>> index, std::size_t side) {
+    return utils::concat("cut-", index_name(index), ":size-", side);
+  }
+
+  static std::string result_space(std::size_t size) { return utils::concat(kResultSpace, size); }
+
+  std::vector<std::string> scratch_list() {
+    std::vector<std::string> names;
+    for (const auto& p : scratch_map_) names.push_back(p.first);
+    return names;
+  }
+
+  const ContractionStats& stats() const { return stats_; }
+
+  // A tensor seen by the interpreter: device data plus index metadata.
+  struct View {
+    const s_type* ptr = nullptr;
+    std::vector<std::string> indices;
+    std::vector<std::size_t> dims;
+    std::size_t size() const;
+  };
+  struct OpenOutput {
+    std::string index;
+    std::size_t count;
+  };
+
+ private:
+  std::vector<Tensor> scratch_;
+  std::unordered_map<std::string, std::size_t> scratch_map_;
+  std::unordered_map<std::string, std::size_t> patch_max_size_;
+  std::vector<std::vector<Tensor>>* tensor_grid_ = nullptr;
+  std::vector<std::complex<double>>* amplitudes_ = nullptr;
+
+  std::vector<std::vector<View>> grid_views_;
+  std::unordered_map<std::string, View> patches_;
+  int depth_ = 0;
+  ContractionStats stats_;
+
+  void run(std::list<ContractionOperation>& ordering, std::size_t output_index,
+           std::unordered_map<std::string, bool>& active_patches,
+           std::vector<OpenOutput> open_outputs);
+  View contract_pair(const View& A, const View& B, s_type* out);
+  View project_view(const View& v, const std::string& index, std::size_t value,
+                    const std::vector<std::size_t>* gather_values, s_type* gather_buffer);
+  void emit_output(const View& output, std::size_t output_index,
+                   const std::vector<OpenOutput>& open_outputs);
+};
+
+void ValidateOrdering(const std::list<ContractionOperation>& ordering);
+
+void ContractGrid(const std::list<ContractionOperation>& ordering,
+                  std::vector<std::vector<Tensor>>* tensor_grid,
+                  std::vector<std::complex<double>>* amplitudes);
+
+// Statistics of the most recent free-function ContractGrid call.
+const ContractionStats& last_contraction_stats();
+
+}  // namespace qflex
+
+#endif

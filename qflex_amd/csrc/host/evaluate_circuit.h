@@ -1,0 +1,34 @@
+// Circuit evaluation entry (interface of reference src/evaluate_circuit.h:
+// get_output_states :63-66, EvaluateCircuit :81-82).
+#ifndef QFLEX_AMD_EVALUATE_CIRCUIT_H_
+#define QFLEX_AMD_EVALUATE_CIRCUIT_H_
+
+#include <complex>
+#include <list>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "contraction_utils.h"
+#include "input.h"
+#include "read_circuit.h"
+#include "tensor.h"
+
+namespace qflex {
+
+std::size_t find_output_pos(const QflexInput* input, std::vector<std::size_t> tensor_pos);
+
+// Output bitstrings of an ordering's terminal cuts: the base final state with
+// terminal-cut positions enumerated in cut order, first cut slowest.
+std::string get_output_states(const QflexInput* input,
+                              const std::list<ContractionOperation>& ordering,
+                              std::vector<std::vector<std::size_t>>* final_qubits,
+                              std::vector<std::string>* output_states);
+
+// Parses the ordering, builds and flattens the network on the device,
+// contracts it and returns (bitstring, amplitude) pairs.
+std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexInput* input);
+
+}  // namespace qflex
+
+#endif

@@ -1,0 +1,22 @@
+// Process-wide settings of the host (reference src/global.h:9-15).
+#ifndef QFLEX_AMD_GLOBAL_H_
+#define QFLEX_AMD_GLOBAL_H_
+
+#include <cstddef>
+
+namespace qflex::global {
+
+// Verbosity of timing / allocation messages on stderr.
+inline int verbose = 0;
+
+// Largest device footprint a contraction may plan (default 1 GiB, as the
+// reference).  Checked by ContractionData::Initialize.
+inline std::size_t memory_limit = std::size_t(1) << 30;
+
+// Arithmetic form of every complex GEMM issued by the host; one of the
+// qfx_precision values of include/qfx.h (0 = FP32 MFMA).
+inline int precision = 0;
+
+}  // namespace qflex::global
+
+#endif

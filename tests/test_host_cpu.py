@@ -1,0 +1,145 @@
+"""Host-side logic of the C++ drop-in that needs no GPU: parsers, ordering
+validation, output-state enumeration and their error texts (restating the
+reference's tests/src/{circuit,grid,ordering,contraction_utils,
+evaluate_circuit}_test.cpp cases)."""
+import json
+import os
+
+import pytest
+
+from qflex_amd import qflex
+
+KATS = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_kats.json")))
+
+
+def ops_from_json(raw):
+    out = []
+    for op in raw:
+        if op[0] == "cut":
+            out.append(("cut", op[1], op[2] if len(op) > 2 else []))
+        else:
+            out.append(tuple(op))
+    return out
+
+
+@pytest.mark.parametrize("name", [n for n in KATS["get_output_states"] if not n.startswith("_")])
+def test_get_output_states(name):
+    c = KATS["get_output_states"][name]
+    base, fq, states = qflex.get_output_states(c["I"], c["J"], c["off"], "", c["final"],
+                                               ops_from_json(c["ops"]))
+    assert fq == c["final_qubits"]
+    assert states == c["states"]
+
+
+def test_get_output_states_ignores_non_terminal_ops():
+    # evaluate_circuit_test.cpp:24-49
+    for ops in ([("expand", "a", [0, 0]), ("expand", "a", [0, 1])],
+                [("merge", "a", "b"), ("merge", "b", "c")],
+                [("cut", [[0, 0], [0, 1]], [1, 2]), ("cut", [[0, 0], [1, 0]], [3])]):
+        base, fq, states = qflex.get_output_states(0, 0, [], "", "", ops)
+        assert fq == [] and states == [""]
+
+
+def test_parse_simple_ordering():
+    # contraction_utils_test.cpp:376-428
+    lines = ["# test comment", "cut (1,2) 1 3", "expand a 1", "expand a 0", "expand a 2",
+             "cut () 5", "expand b 5", "expand b 3", "merge a b"]
+    ops = qflex.parse_ordering(lines, ["11", "11", "01"])
+    assert ops == [("cut", [[0, 1], [1, 1]], [1, 2]), ("expand", "a", [0, 1]),
+                   ("expand", "a", [0, 0]), ("expand", "a", [1, 0]), ("cut", [[2, 1]], []),
+                   ("expand", "b", [2, 1]), ("expand", "b", [1, 1]), ("merge", "a", "b")]
+
+
+def test_parse_inverted_cut_and_spaces():
+    assert qflex.parse_ordering(["cut (1,2) 1 0"], ["11", "11", "01"]) == \
+        [("cut", [[0, 0], [0, 1]], [1, 2])]
+    assert qflex.parse_ordering(["cut   (  1  , 2   )    2        3"], ["11", "11", "01"]) == \
+        [("cut", [[1, 0], [1, 1]], [1, 2])]
+
+
+@pytest.mark.parametrize("line,msg", [
+    ("bad op 1 2", "Received an invalid operation in config."),
+    ("expand a 8", "Index 8 must be within grid boundaries."),
+    ("expand a -2", "Index -2 must be within grid boundaries."),
+    ("cut () 1 7", "Index 7 must be within grid boundaries."),
+    ("cut () -1 4", "Index -1 must be within grid boundaries."),
+    ("cut 2 3", "Cut values must be comma-separated ints, e.g. (0,1,3)."),
+    ("expand a 4", "Index must specify an active qubit."),
+])
+def test_parse_ordering_errors(line, msg):
+    with pytest.raises(RuntimeError, match=msg.replace("(", r"\(").replace(")", r"\)")
+                       .replace(".", r"\.")):
+        qflex.parse_ordering([line], ["11", "11", "01"])
+
+
+def test_validate_ordering_errors():
+    cut = ("cut", [[1, 2], [1, 3]], [])
+    cases = [
+        ([("expand", "a", [1, 2]), ("expand", "b", [1, 2])], "Tensor (1,2) is contracted multiple times."),
+        ([cut, cut], "Index (1,2),(1,3) is cut multiple times."),
+        ([("merge", "a", "c"), ("merge", "a", "b")], "Patch a is merged multiple times."),
+        ([("merge", "a", "b"), ("expand", "a", [1, 2])],
+         "Tensor at (1,2) is added to non-empty patch a after a cut."),
+        ([("expand", "a", [1, 2]), cut, ("expand", "a", [2, 2])],
+         "Tensor at (2,2) is added to non-empty patch a after a cut."),
+        ([("expand", "a", [1, 1]), ("expand", "b", [1, 2]), cut, ("merge", "a", "b")],
+         "Patch a is merged into non-empty patch b after a cut."),
+    ]
+    for ops, msg in cases:
+        with pytest.raises(RuntimeError) as e:
+            qflex.validate_ordering(ops)
+        assert msg in str(e.value), (ops, str(e.value))
+
+
+BAD_CIRCUITS = [
+    ("0 h 0\n0 h 1\n9 h 0\n9 h 1",
+     "[1: 0 h 0] First line in circuit must be the number of active qubits."),
+    ("6\n0 h 0\n0 h\n9 h 0\n9 h 1",
+     "[3: 0 h] Gate must be specified as: cycle gate_name[(p1[,p2,...])] q1 [q2, ...]"),
+    ("2\n0 h 0\n0 h 1\ncz 0 2\n9 h 1", "[4: cz 0 2] First token must be a valid cycle number."),
+    ("2\n0 h 0\n0 h 1\n9 cz 0 2\n7 cz 1 3\n9 h 1", "[5: 7 cz 1 3] Cycle number can only increase."),
+    ("4\n0 h 0\n0 h 1\n1 t 0\n1 t 2\n2 fsim(0.1, a) 0 1\n4 h 0\n4 h 1\n",
+     "Params must be valid numbers."),
+    ("2\n0 h 0\n0 h 1\n1 t 0\n1 t a\n4 h 0\n4 h 1", "[5: 1 t a] Qubit must be a valid number."),
+    ("4\n0 h 0\n0 h 1\n4 t 2\n4 t 4 \n4 cz 2 4\n9 cz 0 2\n9 cz 1 3\n9 h 1",
+     "[6: 4 cz 2 4] Qubits can only used once per cycle."),
+    ("2\n0 h(0 0\n", "Not-matching parentheses."),
+]
+
+
+@pytest.mark.parametrize("text,msg", BAD_CIRCUITS)
+def test_circuit_errors(text, msg):
+    # tests/src/circuit_test.cpp:36-160
+    with pytest.raises(RuntimeError) as e:
+        qflex.load_circuit(text.split("\n"))
+    assert msg in str(e.value)
+
+
+def test_circuit_parse_and_depth():
+    n, depth, gates = qflex.load_circuit(
+        ["3 # qubits", "0 h 0", "0\th 1", "1 fsim(0.25, 0.1) 0 1", "2 cz 1 2 # c", "3 cz 0 1"])
+    assert n == 3 and depth == 3
+    assert gates[1] == (0, "h", [1], [])
+    assert gates[2] == (1, "fsim", [0, 1], [0.25, 0.1])
+
+
+def test_grid_parse_and_error():
+    I, J, off = qflex.load_grid(["1 1 0", "# c", "1 0 1   ", ""])
+    assert (I, J, off) == (2, 3, [[0, 2], [1, 1]])
+    with pytest.raises(RuntimeError, match="Grid size at line 2 is inconsistent with a width of 4 "
+                                           "instead of 3"):
+        qflex.load_grid(["1 1 1", "1 1 1 1", "1 1 1"])
+
+
+def test_simulate_swallows_errors_and_returns_empty(capfd):
+    # src/pybind_main.cpp:127-137: errors are printed, [] is returned
+    out = qflex.simulate({"circuit": ["2", "0 h 0", "0 h 1"], "grid": ["11"],
+                          "ordering": ["expand a 0", "expand a 1"],
+                          "initial_states": ["00", "11"]})
+    assert out == []
+    assert "Not yet supported" in capfd.readouterr().err
+
+
+def test_simulate_rejects_bad_memory_limit(capfd):
+    assert qflex.simulate({"memory_limit": -5}) == []
+    assert "'memory_limit' must be positive." in capfd.readouterr().err
