@@ -107,17 +107,22 @@ def plan(ops, grid, memory_limit=1 << 30):
     return total, pmax
 
 
-def contract_grid(ops, grid, amplitudes):
+def contract_grid(ops, grid, amplitudes, phase_seconds=None):
     """ContractionData::ContractGrid (src/contraction_utils.cpp:223-354):
-    cut recursion, terminal-cut output indexing, FP64 accumulation."""
+    cut recursion, terminal-cut output indexing, FP64 accumulation.
+
+    phase_seconds (optional dict): accumulates the time spent in expand /
+    merge steps by cut depth (number of cuts enclosing the step)."""
+    import time
     patches = {}
 
-    def run(rest, out_idx, active):
+    def run(rest, out_idx, active, depth=0):
         output = None
         k = 0
         while k < len(rest):
             op = rest[k]
             k += 1
+            t_op = time.perf_counter()
             if op[0] == "expand":
                 nxt = grid[op[2][0]][op[2][1]]
                 if not active.get(op[1], False):
@@ -125,6 +130,8 @@ def contract_grid(ops, grid, amplitudes):
                     active[op[1]] = True
                     continue
                 res = multiply(patches[op[1]], nxt)
+                if phase_seconds is not None:
+                    phase_seconds[depth] = phase_seconds.get(depth, 0.0) + time.perf_counter() - t_op
                 if k == len(rest):
                     output = res
                     continue
@@ -141,13 +148,13 @@ def contract_grid(ops, grid, amplitudes):
                     for v in values:
                         grid[pa[0]][pa[1]] = copy_a.project(name, v)
                         grid[pb[0]][pb[1]] = copy_b.project(name, v)
-                        run(suffix, out_idx, dict(active))
+                        run(suffix, out_idx, dict(active), depth + 1)
                     grid[pb[0]][pb[1]] = copy_b
                 else:
                     out_idx *= len(values)
                     for v in values:
                         grid[pa[0]][pa[1]] = copy_a.project(name, v)
-                        run(suffix, out_idx, dict(active))
+                        run(suffix, out_idx, dict(active), depth + 1)
                         out_idx += 1
                 grid[pa[0]][pa[1]] = copy_a
                 return
@@ -158,6 +165,8 @@ def contract_grid(ops, grid, amplitudes):
                     active[tgt] = True
                     continue
                 res = multiply(patches[src], patches[tgt])
+                if phase_seconds is not None:
+                    phase_seconds[depth] = phase_seconds.get(depth, 0.0) + time.perf_counter() - t_op
                 if k == len(rest):
                     output = res
                     continue
@@ -213,8 +222,12 @@ def get_output_states(I, J, off, initial, final, ops):
 
 
 def evaluate_circuit(circuit_text, ordering_text, grid_text, initial="", final="",
-                     dtype=np.complex128, memory_limit=None):
-    """EvaluateCircuit (src/evaluate_circuit.cpp:106-253)."""
+                     dtype=np.complex128, memory_limit=None, phase_seconds=None):
+    """EvaluateCircuit (src/evaluate_circuit.cpp:106-253).  phase_seconds
+    (optional dict) receives "build" (network + flatten) and the per-depth
+    contraction times of contract_grid."""
+    import time
+    t0 = time.perf_counter()
     grid = Grid(grid_text)
     ops = parse_ordering(ordering_lines(ordering_text), grid.I, grid.J, grid.off)
     validate_ordering(ops)
@@ -227,8 +240,10 @@ def evaluate_circuit(circuit_text, ordering_text, grid_text, initial="", final="
     g3 = circuit_to_network(circuit, grid.I, grid.J, initial, base, final_qubits,
                             grid.off, dtype)
     g2 = flatten(g3, grid.I, grid.J, final_qubits, grid.off, ops)
+    if phase_seconds is not None:
+        phase_seconds["build"] = time.perf_counter() - t0
     if memory_limit is not None:
         plan(ops, g2, memory_limit)
     amps = np.zeros(len(states), dtype=np.complex128)
-    contract_grid(ops, g2, amps)
+    contract_grid(ops, g2, amps, phase_seconds)
     return list(zip(states, amps))

@@ -165,9 +165,14 @@ __global__ __launch_bounds__(256) void cgemm_f32_mfma_kernel(const float2* __res
 }
 
 // ------------------------------------------------------------ split-BF16 MFMA
-// Same block tile; LDS holds FP32 (A rows interleaved, B complex) and the
-// split happens when operands are read: v_mfma_f32_32x32x16_bf16 consumes 8
-// complex k per instruction.
+// Same block tile as the FP32 kernel.  Operands are split once, while they
+// are staged into LDS, into NT BF16 planes (hi, mid[, lo]):
+//   As[t][row][32 reals]          A rows, interleaved (re, im) along k;
+//   Bs[t][part*64 + n][32 reals]  Bhat columns: part 0 (the Cr column of
+//                                 complex column n) holds (Br_k, -Bi_k),
+//                                 part 1 (Ci) holds (Bi_k, Br_k).
+// v_mfma_f32_32x32x16_bf16 then reads 8 consecutive BF16 of a row per lane
+// (one ds_read_b128) and consumes 8 complex k per instruction.
 __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l) {
   h = static_cast<__bf16>(x);
   float r = x - static_cast<float>(h);
@@ -175,15 +180,16 @@ __device__ __forceinline__ void split3(float x, __bf16& h, __bf16& m, __bf16& l)
   l = static_cast<__bf16>(r - static_cast<float>(m));
 }
 
-constexpr int kAStrideB = 2 * kBK + 4;  // 16-byte aligned rows for b128 reads
+constexpr int kRowB = 40;  // BF16 per LDS row: 32 + 8 pad (80 B, 16-byte aligned)
 
 template <int TERMS>
 __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __restrict__ A,
                                                               const float2* __restrict__ B,
                                                               float2* __restrict__ C, int M,
                                                               int N, int K) {
-  __shared__ __align__(16) float As[2][kBM * kAStrideB];
-  __shared__ __align__(16) float2 Bs[2][kBK * kBN];
+  constexpr int NT = TERMS >= 6 ? 3 : 2;
+  __shared__ __align__(16) __bf16 As[NT][kBM * kRowB];
+  __shared__ __align__(16) __bf16 Bs[NT][2 * kBN * kRowB];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -211,22 +217,41 @@ __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __re
       rb[h] = (gk < K && gn < N) ? B[uint64_t(gk) * N + gn] : make_float2(0.f, 0.f);
     }
   };
-  auto store_tile = [&](int buf) {
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  auto store_tile = [&]() {
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
       int r = (tid >> 4) + 16 * h;
       int kk = tid & 15;
-      *reinterpret_cast<float2*>(&As[buf][r * kAStrideB + 2 * kk]) = ra[h];
+      __bf16 xh, xm, xl, yh, ym, yl;
+      split3(ra[h].x, xh, xm, xl);
+      split3(ra[h].y, yh, ym, yl);
+      __bf16 th[3][2] = {{xh, yh}, {xm, ym}, {xl, yl}};
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bf16x2 v = {th[t][0], th[t][1]};
+        *reinterpret_cast<bf16x2*>(&As[t][r * kRowB + 2 * kk]) = v;
+      }
     }
 #pragma unroll
     for (int h = 0; h < 4; ++h) {
       int kk = (tid >> 6) + 4 * h;
       int nn = tid & 63;
-      Bs[buf][kk * kBN + nn] = rb[h];
+      __bf16 rh, rm, rl, ih, im, il;
+      split3(rb[h].x, rh, rm, rl);
+      split3(rb[h].y, ih, im, il);
+      __bf16 tr[3] = {rh, rm, rl}, ti[3] = {ih, im, il};
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bf16x2 c0 = {tr[t], static_cast<__bf16>(-static_cast<float>(ti[t]))};
+        bf16x2 c1 = {ti[t], tr[t]};
+        *reinterpret_cast<bf16x2*>(&Bs[t][nn * kRowB + 2 * kk]) = c0;
+        *reinterpret_cast<bf16x2*>(&Bs[t][(kBN + nn) * kRowB + 2 * kk]) = c1;
+      }
     }
   };
 
-  const int half = lane >> 5;  // lanes 0-31: complex k 0..3 of the 8; 32-63: k 4..7
+  const int half = lane >> 5;
   const int col = lane & 31;
   const int part = col & 1;
 
@@ -240,54 +265,25 @@ __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __re
 
   const int nkt = (K + kBK - 1) / kBK;
   load_tile(0);
-  store_tile(0);
-  __syncthreads();
   for (int kt = 0; kt < nkt; ++kt) {
-    const int buf = kt & 1;
+    __syncthreads();
+    store_tile();
+    __syncthreads();
     if (kt + 1 < nkt) load_tile(kt + 1);
-    const float* as = As[buf];
-    const float2* bs = Bs[buf];
 #pragma unroll
     for (int ks = 0; ks < kBK; ks += 8) {
-      // A operand: row (lane%32), reals 8*half..8*half+7 of this 8-k step,
-      // i.e. complex k = ks + 4*half .. +3 interleaved (re, im).
-      bf16x8 ah[2], am[2], al[2];
+      const int off = 2 * ks + 8 * half;
+      bf16x8 a[NT][2], b[NT][2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const float* ap = as + (wm * 64 + i * 32 + col) * kAStrideB + 2 * (ks + 4 * half);
-        float4 v0 = *reinterpret_cast<const float4*>(ap);
-        float4 v1 = *reinterpret_cast<const float4*>(ap + 4);
-        float vals[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      for (int t = 0; t < NT; ++t) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          __bf16 h, m, l;
-          split3(vals[e], h, m, l);
-          ah[i][e] = h;
-          am[i][e] = m;
-          al[i][e] = l;
-        }
-      }
-      // B operand: Bhat rows 8*half..8*half+7 (complex k = ks+4*half+q,
-      // q = 0..3, re row then im row), real column col of the 32-wide tile.
-      bf16x8 bh[2], bm[2], bl[2];
+        for (int i = 0; i < 2; ++i)
+          a[t][i] = *reinterpret_cast<const bf16x8*>(&As[t][(wm * 64 + i * 32 + col) * kRowB + off]);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int nn = wn * 32 + j * 16 + (col >> 1);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          float2 b = bs[(ks + 4 * half + q) * kBN + nn];
-          // re row: (Br, Bi) for (even, odd) columns; im row: (-Bi, Br)
-          float v_re = part ? b.y : b.x;
-          float v_im = part ? b.x : -b.y;
-          __bf16 h, m, l;
-          split3(v_re, h, m, l);
-          bh[j][2 * q] = h;
-          bm[j][2 * q] = m;
-          bl[j][2 * q] = l;
-          split3(v_im, h, m, l);
-          bh[j][2 * q + 1] = h;
-          bm[j][2 * q + 1] = m;
-          bl[j][2 * q + 1] = l;
+        for (int j = 0; j < 2; ++j) {
+          const int rc = wn * 64 + j * 32 + col;  // real column of Bhat in this block
+          const int row = (rc & 1) * kBN + (rc >> 1);
+          b[t][j] = *reinterpret_cast<const bf16x8*>(&Bs[t][row * kRowB + off]);
         }
       }
 #pragma unroll
@@ -295,17 +291,15 @@ __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __re
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           if (TERMS >= 6) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[i], bm[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
           }
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(am[i], bh[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bm[j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
         }
     }
-    if (kt + 1 < nkt) store_tile(buf ^ 1);
-    __syncthreads();
   }
 
   float* Cf = reinterpret_cast<float*>(C);
@@ -356,33 +350,70 @@ __global__ __launch_bounds__(256) void gemv_n_kernel(const float2* __restrict__ 
   }
 }
 
-// Partial y over a chunk of rows: part[s][c] = sum_{r in chunk s} x[r] A[r][c].
+// y[c] = sum_r x[r] A[r][c] (A is m x k, row-major), split over rows.
+// Stage 1: block s reduces rows [s*rows_per, ...) for every column into
+// part[s][c]: 64 column lanes x 4 row groups, fixed order, LDS tree.
 __global__ __launch_bounds__(256) void gemv_t_partial_kernel(const float2* __restrict__ A,
                                                              const float2* __restrict__ x,
                                                              float2* __restrict__ part,
                                                              uint64_t M, uint64_t K,
                                                              uint64_t rows_per_split) {
-  uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  uint64_t s = blockIdx.y;
-  if (c >= K) return;
-  uint64_t r0 = s * rows_per_split;
-  uint64_t r1 = r0 + rows_per_split < M ? r0 + rows_per_split : M;
-  float2 acc = make_float2(0.f, 0.f);
-  for (uint64_t r = r0; r < r1; ++r) acc = cfma(x[r], A[r * K + c], acc);
-  part[s * K + c] = acc;
+  __shared__ float2 red[4][64];
+  const uint64_t s = blockIdx.x;
+  const int cl = threadIdx.x & 63;
+  const int rg = threadIdx.x >> 6;
+  const uint64_t r0 = s * rows_per_split;
+  const uint64_t r1 = r0 + rows_per_split < M ? r0 + rows_per_split : M;
+  for (uint64_t c0 = 0; c0 < K; c0 += 64) {
+    const uint64_t c = c0 + cl;
+    float2 acc = make_float2(0.f, 0.f);
+    if (c < K)
+      for (uint64_t r = r0 + rg; r < r1; r += 4) acc = cfma(x[r], A[r * K + c], acc);
+    red[rg][cl] = acc;
+    __syncthreads();
+    if (rg == 0 && c < K) {
+      float2 t = red[0][cl];
+#pragma unroll
+      for (int g = 1; g < 4; ++g) {
+        t.x += red[g][cl].x;
+        t.y += red[g][cl].y;
+      }
+      part[s * K + c] = t;
+    }
+    __syncthreads();
+  }
 }
 
-__global__ void sum_partials_kernel(const float2* __restrict__ part, float2* __restrict__ y,
-                                    uint64_t nsplit, uint64_t K) {
-  uint64_t c = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= K) return;
-  float2 acc = make_float2(0.f, 0.f);
-  for (uint64_t s = 0; s < nsplit; ++s) {
-    float2 v = part[s * K + c];
-    acc.x += v.x;
-    acc.y += v.y;
+// Stage 2: y[c] = sum_s part[s][c]; 64 columns x 4 split groups per block,
+// grid-stride over column tiles.
+__global__ __launch_bounds__(256) void sum_partials_kernel(const float2* __restrict__ part,
+                                                           float2* __restrict__ y,
+                                                           uint64_t nsplit, uint64_t K) {
+  __shared__ float2 red[4][64];
+  const int cl = threadIdx.x & 63;
+  const int sg = threadIdx.x >> 6;
+  for (uint64_t c0 = uint64_t(blockIdx.x) * 64; c0 < K; c0 += uint64_t(gridDim.x) * 64) {
+    const uint64_t c = c0 + cl;
+    float2 acc = make_float2(0.f, 0.f);
+    if (c < K)
+      for (uint64_t s = sg; s < nsplit; s += 4) {
+        float2 v = part[s * K + c];
+        acc.x += v.x;
+        acc.y += v.y;
+      }
+    red[sg][cl] = acc;
+    __syncthreads();
+    if (sg == 0 && c < K) {
+      float2 t = red[0][cl];
+#pragma unroll
+      for (int g = 1; g < 4; ++g) {
+        t.x += red[g][cl].x;
+        t.y += red[g][cl].y;
+      }
+      y[c] = t;
+    }
+    __syncthreads();
   }
-  y[c] = acc;
 }
 
 // Block partial sums of x[i]*y[i] in a fixed per-lane order, then a tree.
@@ -414,19 +445,16 @@ unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<u
 
 static int launch_gemv_t(qfx_context* ctx, uint64_t m, uint64_t k, const float2* A,
                          const float2* x, float2* y) {
-  // Split the m rows so that about 2048 blocks of 256 columns are in flight.
-  uint64_t col_blocks = (k + 255) / 256;
-  uint64_t nsplit = std::max<uint64_t>(1, std::min<uint64_t>(m, 2048 / col_blocks));
-  nsplit = std::min<uint64_t>(nsplit, 65535);
+  // About 1024 row blocks of at least 16 rows each.
+  uint64_t nsplit = std::max<uint64_t>(1, std::min<uint64_t>(1024, (m + 15) / 16));
   uint64_t rows_per = (m + nsplit - 1) / nsplit;
   nsplit = (m + rows_per - 1) / rows_per;
   void* ws = nullptr;
   int st = qfx_workspace(ctx, nsplit * k * sizeof(float2), &ws);
   if (st != QFX_OK) return st;
   float2* part = static_cast<float2*>(ws);
-  dim3 g{unsigned(col_blocks), unsigned(nsplit), 1};
-  gemv_t_partial_kernel<<<g, 256, 0, ctx->stream>>>(A, x, part, m, k, rows_per);
-  sum_partials_kernel<<<cap_grid((k + 255) / 256), 256, 0, ctx->stream>>>(part, y, nsplit, k);
+  gemv_t_partial_kernel<<<unsigned(nsplit), 256, 0, ctx->stream>>>(A, x, part, m, k, rows_per);
+  sum_partials_kernel<<<cap_grid((k + 63) / 64), 256, 0, ctx->stream>>>(part, y, nsplit, k);
   QFX_HIP(hipGetLastError());
   return QFX_OK;
 }
