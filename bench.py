@@ -78,8 +78,9 @@ def time_variant(qflex, opts, steps, warmup):
     qflex.profile_reset()
     qflex.synchronize()
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         r = qflex.evaluate(opts)
+        log(f"[bench]   step {k + 1}/{steps} done")
     qflex.synchronize()
     dt = time.perf_counter() - t0
     prof = qflex.profile_read()
@@ -136,12 +137,25 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--variants", default="fp32,split_bf16x3,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="first run one device copy of this many MiB (qfx_copy_kernel), the "
+                         "known-size reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
     from qflex_amd import qflex
+
+    if args.calibrate_copy_mib:
+        from qflex_amd import capi
+        ctx = capi.Context(0)
+        n = args.calibrate_copy_mib << 20
+        a, b = ctx.alloc(n), ctx.alloc(n)
+        ctx.copy(b, a, n)
+        ctx.sync()
+        log(f"[bench] calibration copy: {n} bytes read, {n} bytes written")
+        ctx.close()
 
     circ, order, grid = workload(args.seed)
     variants = {}

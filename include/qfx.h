@@ -48,7 +48,7 @@ enum qfx_status {
 /* Arithmetic forms of the complex GEMM. */
 enum qfx_precision {
   QFX_PREC_FP32 = 0,       /* FP32 MFMA, FP32 accumulate */
-  QFX_PREC_SPLIT_BF16 = 1, /* A,B = hi+lo BF16; hi*hi + hi*lo + lo*hi, FP32 acc */
+  QFX_PREC_SPLIT_BF16 = 1, /* A,B = hi+mid BF16; hi*hi + hi*mid + mid*hi, FP32 acc */
   QFX_PREC_FP32_EXACT = 2, /* scalar fmaf in K order (defined reduction order) */
   QFX_PREC_SPLIT_BF16X3 = 3 /* A,B = hi+mid+lo BF16; 6 cross products, FP32 acc */
 };
@@ -56,12 +56,13 @@ enum qfx_precision {
 /* Kernel classes reported by qfx_profile_read. */
 enum qfx_kernel_class {
   QFX_K_PERMUTE = 0,
-  QFX_K_CGEMM = 1,
+  QFX_K_CGEMM = 1,        /* MFMA complex GEMM kernels */
   QFX_K_GEMV = 2,
   QFX_K_DOT = 3,
   QFX_K_ACCUMULATE = 4,
   QFX_K_COPY = 5,
-  QFX_K_NUM_CLASSES = 6
+  QFX_K_CGEMM_SMALL = 6,  /* scalar complex GEMM (small shapes, FP32_EXACT) */
+  QFX_K_NUM_CLASSES = 7
 };
 
 /* Message of the last failed call on this host thread ("" if none). */

@@ -13,8 +13,8 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libq
 FP32, SPLIT_BF16, FP32_EXACT, SPLIT_BF16X3 = 0, 1, 2, 3
 PRECISIONS = {"fp32": FP32, "split_bf16": SPLIT_BF16, "fp32_exact": FP32_EXACT,
               "split_bf16x3": SPLIT_BF16X3}
-K_PERMUTE, K_CGEMM, K_GEMV, K_DOT, K_ACCUMULATE, K_COPY = range(6)
-KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy"]
+K_PERMUTE, K_CGEMM, K_GEMV, K_DOT, K_ACCUMULATE, K_COPY, K_CGEMM_SMALL = range(7)
+KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy", "cgemm_small"]
 
 _c = ctypes
 _vp = _c.c_void_p
@@ -113,6 +113,9 @@ class Context:
         check(load().qfx_copy_d2h(self.handle, out.ctypes.data, buf.ptr, out.nbytes))
         return out
 
+    def copy(self, dst, src, nbytes):
+        check(load().qfx_copy_d2d(self.handle, dst.ptr, src.ptr, nbytes))
+
     def sync(self):
         check(load().qfx_synchronize(self.handle))
 
@@ -142,7 +145,7 @@ class Context:
         check(load().qfx_profile_reset(self.handle))
 
     def profile_read(self):
-        n = 6
+        n = len(KERNEL_CLASSES)
         la = (ctypes.c_uint64 * n)()
         ms = (ctypes.c_double * n)()
         by = (ctypes.c_double * n)()

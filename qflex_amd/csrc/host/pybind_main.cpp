@@ -430,7 +430,8 @@ PYBIND11_MODULE(qflex, m) {
       uint64_t la[QFX_K_NUM_CLASSES];
       double ms[QFX_K_NUM_CLASSES], by[QFX_K_NUM_CLASSES], fl[QFX_K_NUM_CLASSES];
       qflex::qfx_check(qfx_profile_read(qflex::device(), la, ms, by, fl), "qfx_profile_read");
-      const char* names[] = {"permute", "cgemm", "gemv", "dot", "accumulate", "copy"};
+      const char* names[] = {"permute", "cgemm",  "gemv",       "dot",
+                             "accumulate", "copy", "cgemm_small"};
       py::dict d;
       for (int i = 0; i < QFX_K_NUM_CLASSES; ++i) {
         py::dict e;

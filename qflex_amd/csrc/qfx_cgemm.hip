@@ -527,7 +527,7 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
       // y[n] = sum_k a[k] B[k][n]: transposed GEMV of B (k x n) with x = A.
       return launch_gemv_t(ctx, k, n, B2, A2, C2);
     }
-    QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     cgemm_scalar_kernel<<<cap_grid((m * n + 255) / 256), 256, 0, ctx->stream>>>(A2, B2, C2, m, n,
                                                                                k);
     QFX_HIP(hipGetLastError());
