@@ -61,8 +61,9 @@ def measure_error(qflex, circ, grid, precision):
                    key=lambda p: json.load(open(p))["slices"])
     if not files:
         return None, 0
+    from qflex_amd import circuits
     ref = json.load(open(files[-1]))
-    order = "\n".join(ref["ordering"]) + "\n"
+    order = circuits.corner_inward_7x7(tuple(ref["cut_values"]))
     r = qflex.evaluate(options(circ, order, grid, precision))
     got = {s: a for s, a in r}
     refs = [complex(re, im) for _, re, im in ref["amplitudes"]]
@@ -90,9 +91,11 @@ def time_variant(qflex, opts, steps, warmup):
 
 def cpu_baseline(circ, grid):
     """The oracle in complex64 (host reorders + numpy/OpenBLAS cgemm, the
-    reference's algorithm) on a bounded sample: the pre-cut work, one slice
-    and one terminal value, each phase timed, extrapolated to the full job
-    (pre-cut once, cut-1 phase x16, cut-2 phase x256, terminal suffix x16384)."""
+    reference's algorithm, terminal cuts by recursion) on a bounded sample:
+    the pre-cut work and one of the 256 slices with all 64 outputs.  Each
+    phase is timed by cut depth and scaled by how many more times the full
+    job runs it (x1 before the first cut, x16 after it, x256 after the
+    second), i.e. the full-job time is extrapolated."""
     import numpy as np
     from oracle.contraction import evaluate_circuit
     from qflex_amd import circuits
@@ -102,31 +105,63 @@ def cpu_baseline(circ, grid):
                      if i.get("internal_api") == "openblas"] or [os.cpu_count()])
     except Exception:
         cores = os.cpu_count()
-    lines = circuits.corner_inward_7x7().splitlines()
-    counts = []  # value count of each cut, in order, in the full job
-    sample = []
-    for l in lines:
-        if l.startswith("cut ()"):
-            counts.append(16 if len(l.split()) == 4 else 2)
-            l = l.replace("()", "(0)")
-        sample.append(l)
+    lines = circuits.corner_inward_7x7(([0], [0])).splitlines()
+    # value count of each cut in the full job / in the sample, in order
+    scale = [16 if len(l.split()) == 4 else 1 for l in lines if l.startswith("cut ")]
     phases = {}
     t0 = time.perf_counter()
-    evaluate_circuit(circ, "\n".join(sample) + "\n", grid, "0" * 49, "", dtype=np.complex64,
+    evaluate_circuit(circ, "\n".join(lines) + "\n", grid, "0" * 49, "", dtype=np.complex64,
                      phase_seconds=phases)
     wall = time.perf_counter() - t0
     total = phases.pop("build", 0.0)
     for depth, sec in phases.items():
         mult = 1
-        for c in counts[:depth]:
+        for c in scale[:depth]:
             mult *= c
         total += sec * mult
     return {"value": 64.0 / total, "unit": "amplitudes/s", "cores": int(cores), "kind": "port",
             "sample": (f"oracle complex64 (numpy/OpenBLAS cgemm + host reorders, the reference "
-                       f"algorithm incl. terminal-cut recursion): pre-cut work + 1 of 256 slices + "
-                       f"1 of 64 outputs, {wall:.1f} s wall; per-depth times extrapolated to the "
-                       f"full 256-slice x 64-output job = {total:.0f} s"),
+                       f"algorithm incl. terminal-cut recursion): pre-cut work + 1 of 256 slices "
+                       f"with all 64 outputs, {wall:.1f} s wall; per-depth times extrapolated to "
+                       f"the full 256-slice job = {total:.0f} s"),
             "extrapolated": True}
+
+
+def microbench(args):
+    """Times qfx_cgemm_c64 on fixed shapes (random complex N(0,1) operands)."""
+    import numpy as np
+    from qflex_amd import capi
+    ctx = capi.Context(0)
+    rng = np.random.default_rng(0)
+    out = []
+    for shape in args.microbench.split(";"):
+        m, n, k = (int(x) for x in shape.split(","))
+        a = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))).astype(np.complex64)
+        b = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex64)
+        A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+        for prec in args.variants.split(","):
+            p = capi.PRECISIONS[prec]
+            for _ in range(args.warmup):
+                ctx.cgemm(p, m, n, k, A, B, C)
+            ctx.profile(True)
+            ctx.profile_reset()
+            for _ in range(args.steps):
+                ctx.cgemm(p, m, n, k, A, B, C)
+            prof = ctx.profile_read()
+            ctx.profile(False)
+            cls = "cgemm" if prof["cgemm"]["launches"] else ("gemv" if prof["gemv"]["launches"]
+                                                             else "cgemm_small")
+            g = prof[cls]
+            ms = g["ms"] / max(g["launches"], 1)
+            out.append({"m": m, "n": n, "k": k, "variant": prec, "class": cls, "avg_ms": ms,
+                        "tflops": 8.0 * m * n * k / ms / 1e9,
+                        "frac_of_peak": 8.0 * m * n * k / ms / 1e9 / PEAK_TFLOPS[prec]
+                        if prec in PEAK_TFLOPS else None})
+            log(f"[bench] {m}x{n}x{k} {prec}: {ms:.3f} ms, {out[-1]['tflops']:.1f} TF/s")
+        for buf in (A, B, C):
+            buf.free()
+    ctx.close()
+    return {"microbench": out}
 
 
 def main():
@@ -137,6 +172,10 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--variants", default="fp32,split_bf16x3,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--microbench", default="",
+                    help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
+                         "with HIP events through the C-ABI, each --steps times per variant); "
+                         "prints one JSON line and skips the workload")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device copy of this many MiB (qfx_copy_kernel), the "
                          "known-size reference for HBM byte counters")
@@ -156,6 +195,10 @@ def main():
         ctx.sync()
         log(f"[bench] calibration copy: {n} bytes read, {n} bytes written")
         ctx.close()
+
+    if args.microbench:
+        print(json.dumps(microbench(args)))
+        return
 
     circ, order, grid = workload(args.seed)
     variants = {}

@@ -315,6 +315,183 @@ __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __re
       }
 }
 
+// ------------------------------------------------------------ split-BF16, large tiles
+// For m >= 256, n >= 128 (k, n even).  512 threads (8 waves as 4 x 2), block
+// tile 256 complex rows x 128 complex columns, each wave 64 rows x 64
+// complex columns = 2 x 4 MFMA tiles of 32 x 32 (128 accumulator registers).
+// One stage = 8 complex k = 16 reals = one v_mfma_f32_32x32x16_bf16 k-step.
+// LDS holds two stages of NT BF16 planes:
+//   As[buf][t][row][16 reals + 8 pad]      (48-byte rows: conflict-free b128)
+//   Bs[buf][t][part*128 + n][16 + 8 pad]   Bhat columns, as in the kernel above
+// The next stage's global loads are issued before the MFMAs of the current
+// one; its split and LDS stores follow the MFMAs; one barrier per stage.
+constexpr int kBBM = 256, kBBN = 128, kBBK = 8;
+constexpr int kRowL = 24;  // BF16 per LDS row
+
+template <int TERMS>
+__global__ __launch_bounds__(512) void cgemm_bf16_big_kernel(const float2* __restrict__ A,
+                                                             const float2* __restrict__ B,
+                                                             float2* __restrict__ C, int M,
+                                                             int N, int K) {
+  constexpr int NT = TERMS >= 6 ? 3 : 2;
+  extern __shared__ __align__(16) __bf16 smem[];
+  // [buf][plane][rows][kRowL]: A planes then B planes per buffer
+  constexpr int kAPlane = kBBM * kRowL;
+  constexpr int kBPlane = 2 * kBBN * kRowL;
+  constexpr int kBuf = NT * (kAPlane + kBPlane);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1;  // 0..3: 64 rows each
+  const int wn = wave & 1;   // 0..1: 64 complex columns each
+  const int m0 = blockIdx.x * kBBM;
+  const int n0 = blockIdx.y * kBBN;
+
+  // global -> register assignment
+  // A: 256 rows x 8 k = 1024 float4 (2 complex each); f = tid + 512*h:
+  //    row = f >> 2, q = f & 3 (complex k = 2q, 2q+1)
+  // B: 8 k x 128 n = 512 float4; lanes with consecutive tid take
+  //    consecutive k so the LDS stores are conflict-free: k = tid & 7,
+  //    nq = tid >> 3 (complex n = 2nq, 2nq+1)
+  float4 ra[2], rb;
+  auto load_tile = [&](int kt) {
+    const int k0 = kt * kBBK;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int f = tid + 512 * h;
+      const int row = f >> 2, q = f & 3;
+      const int gm = m0 + row, gk = k0 + 2 * q;
+      ra[h] = (gm < M && gk < K) ? *reinterpret_cast<const float4*>(A + uint64_t(gm) * K + gk)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    {
+      const int kk = tid & 7, nq = tid >> 3;
+      const int gk = k0 + kk, gn = n0 + 2 * nq;
+      rb = (gk < K && gn < N) ? *reinterpret_cast<const float4*>(B + uint64_t(gk) * N + gn)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  auto store_tile = [&](int buf) {
+    __bf16* base = smem + buf * kBuf;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int f = tid + 512 * h;
+      const int row = f >> 2, q = f & 3;
+      float v[4] = {ra[h].x, ra[h].y, ra[h].z, ra[h].w};
+      __bf16 t3[3][4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) split3(v[e], t3[0][e], t3[1][e], t3[2][e]);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        bf16x4 w = {t3[t][0], t3[t][1], t3[t][2], t3[t][3]};
+        *reinterpret_cast<bf16x4*>(base + t * kAPlane + row * kRowL + 4 * q) = w;
+      }
+    }
+    {
+      const int kk = tid & 7, nq = tid >> 3;
+      __bf16* bb = base + NT * kAPlane;
+      float v[4] = {rb.x, rb.y, rb.z, rb.w};  // (Br, Bi) of n, then of n+1
+      __bf16 t3[3][4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) split3(v[e], t3[0][e], t3[1][e], t3[2][e]);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const int n = 2 * nq + c;
+          const __bf16 br = t3[t][2 * c], bi = t3[t][2 * c + 1];
+          bf16x2 p0 = {br, static_cast<__bf16>(-static_cast<float>(bi))};
+          bf16x2 p1 = {bi, br};
+          *reinterpret_cast<bf16x2*>(bb + t * kBPlane + n * kRowL + 2 * kk) = p0;
+          *reinterpret_cast<bf16x2*>(bb + t * kBPlane + (kBBN + n) * kRowL + 2 * kk) = p1;
+        }
+      }
+    }
+  };
+
+  const int half = lane >> 5;
+  const int col = lane & 31;
+  const int part = col & 1;
+
+  floatx16 acc[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  const int nkt = (K + kBBK - 1) / kBBK;
+  load_tile(0);
+  store_tile(0);
+  __syncthreads();
+  for (int kt = 0; kt < nkt; ++kt) {
+    const int buf = kt & 1;
+    if (kt + 1 < nkt) load_tile(kt + 1);
+    const __bf16* base = smem + buf * kBuf;
+    const __bf16* bb = base + NT * kAPlane;
+    bf16x8 a[NT][2], b[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+        a[t][i] = *reinterpret_cast<const bf16x8*>(base + t * kAPlane +
+                                                   (wm * 64 + i * 32 + col) * kRowL + 8 * half);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int rc = wn * 128 + j * 32 + col;  // real column of Bhat in the block
+        const int row = (rc & 1) * kBBN + (rc >> 1);
+        b[t][j] = *reinterpret_cast<const bf16x8*>(bb + t * kBPlane + row * kRowL + 8 * half);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        if (TERMS >= 6) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+        }
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+      }
+    if (kt + 1 < nkt) store_tile(buf ^ 1);
+    __syncthreads();
+  }
+
+  float* Cf = reinterpret_cast<float*>(C);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+        const int cn = n0 + wn * 64 + j * 16 + (col >> 1);
+        if (row < M && cn < N) Cf[(uint64_t(row) * N + cn) * 2 + part] = acc[i][j][r];
+      }
+}
+
+template <int TERMS>
+size_t bf16_big_smem() {
+  constexpr int NT = TERMS >= 6 ? 3 : 2;
+  return size_t(2) * NT * (kBBM * kRowL + 2 * kBBN * kRowL) * sizeof(__bf16);
+}
+
+// Dynamic LDS above 64 KiB must be allowed per kernel once per process.
+template <int TERMS>
+hipError_t bf16_big_prepare() {
+  static hipError_t e = hipFuncSetAttribute(
+      reinterpret_cast<const void*>(&cgemm_bf16_big_kernel<TERMS>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, int(bf16_big_smem<TERMS>()));
+  return e;
+}
+
 // ------------------------------------------------------------ GEMV / dot
 __device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c) {
   c.x = fmaf(a.x, b.x, c.x);
@@ -439,6 +616,38 @@ __global__ __launch_bounds__(256) void dot_partial_kernel(const float2* __restri
   }
 }
 
+// Few outputs, long k: C[m x n] with m*n <= 4096.  Block (x = k chunk,
+// y = tile of 64 outputs): 4 groups of 64 lanes split the chunk, fixed
+// order, LDS tree; part[chunk][o] is then summed over chunks by
+// sum_partials_kernel.
+__global__ __launch_bounds__(256) void cgemm_splitk_partial_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ part,
+    uint64_t M, uint64_t N, uint64_t K, uint64_t chunk) {
+  __shared__ float2 red[4][64];
+  const int ol = threadIdx.x & 63;
+  const int kg = threadIdx.x >> 6;
+  const uint64_t o = uint64_t(blockIdx.y) * 64 + ol;
+  const uint64_t k0 = uint64_t(blockIdx.x) * chunk;
+  const uint64_t k1 = k0 + chunk < K ? k0 + chunk : K;
+  float2 acc = make_float2(0.f, 0.f);
+  if (o < M * N) {
+    const uint64_t m = o / N, n = o - m * N;
+    const float2* a = A + m * K;
+    for (uint64_t k = k0 + kg; k < k1; k += 4) acc = cfma(a[k], B[k * N + n], acc);
+  }
+  red[kg][ol] = acc;
+  __syncthreads();
+  if (kg == 0 && o < M * N) {
+    float2 t = red[0][ol];
+#pragma unroll
+    for (int g = 1; g < 4; ++g) {
+      t.x += red[g][ol].x;
+      t.y += red[g][ol].y;
+    }
+    part[uint64_t(blockIdx.x) * M * N + o] = t;
+  }
+}
+
 unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, 65535))); }
 
 }  // namespace
@@ -527,6 +736,23 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
       // y[n] = sum_k a[k] B[k][n]: transposed GEMV of B (k x n) with x = A.
       return launch_gemv_t(ctx, k, n, B2, A2, C2);
     }
+    if (precision != QFX_PREC_FP32_EXACT && m * n <= 4096 && k >= 1024) {
+      QfxLaunchScope scope(ctx, QFX_K_GEMV, bytes, flops);
+      const uint64_t outs = m * n;
+      uint64_t nchunk = std::max<uint64_t>(1, std::min<uint64_t>(2048 / ((outs + 63) / 64), k / 256));
+      const uint64_t chunk = (k + nchunk - 1) / nchunk;
+      nchunk = (k + chunk - 1) / chunk;
+      void* ws = nullptr;
+      int st = qfx_workspace(ctx, nchunk * outs * sizeof(float2), &ws);
+      if (st != QFX_OK) return st;
+      float2* part = static_cast<float2*>(ws);
+      dim3 g{unsigned(nchunk), unsigned((outs + 63) / 64), 1};
+      cgemm_splitk_partial_kernel<<<g, 256, 0, ctx->stream>>>(A2, B2, part, m, n, k, chunk);
+      sum_partials_kernel<<<cap_grid((outs + 63) / 64), 256, 0, ctx->stream>>>(part, C2, nchunk,
+                                                                            outs);
+      QFX_HIP(hipGetLastError());
+      return QFX_OK;
+    }
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     cgemm_scalar_kernel<<<cap_grid((m * n + 255) / 256), 256, 0, ctx->stream>>>(A2, B2, C2, m, n,
                                                                                k);
@@ -536,6 +762,22 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
   dim3 grid(unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN));
   if (grid.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
+  const bool big_tile = m >= 1024 && n >= 1024 && (k % 2 == 0) && (n % 2 == 0) &&
+                        (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
+                        (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+  if (precision != QFX_PREC_FP32 && big_tile) {
+    dim3 g2{unsigned((m + kBBM - 1) / kBBM), unsigned((n + kBBN - 1) / kBBN), 1};
+    if (g2.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
+    QFX_HIP(precision == QFX_PREC_SPLIT_BF16 ? bf16_big_prepare<3>() : bf16_big_prepare<6>());
+    if (precision == QFX_PREC_SPLIT_BF16)
+      cgemm_bf16_big_kernel<3><<<g2, 512, bf16_big_smem<3>(), ctx->stream>>>(A2, B2, C2, int(m),
+                                                                            int(n), int(k));
+    else
+      cgemm_bf16_big_kernel<6><<<g2, 512, bf16_big_smem<6>(), ctx->stream>>>(A2, B2, C2, int(m),
+                                                                            int(n), int(k));
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
   if (precision == QFX_PREC_FP32)
     cgemm_f32_mfma_kernel<<<grid, 256, 0, ctx->stream>>>(A2, B2, C2, int(m), int(n), int(k));
   else if (precision == QFX_PREC_SPLIT_BF16)

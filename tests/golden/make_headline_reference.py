@@ -7,9 +7,11 @@ their norm-wise relative error without re-running the CPU oracle:
     python tests/golden/make_headline_reference.py --slices 4    # ~1 min
     python tests/golden/make_headline_reference.py --slices 256  # full job
 
---slices S restricts the second cut to values 0..S/16-1 (and the first cut
-to 0) when S < 256, i.e. the amplitudes are partial sums over S slices; the
-restricted ordering text is stored next to them.
+--slices S < 256 restricts the bond cuts to a subset of their values: the
+amplitudes are then partial sums over S slices.  The subsets are stored
+("cut_values"); amplitudes do not depend on the ordering beyond which
+slices are summed, so any ordering with the same two cuts and the same
+value subsets must reproduce them.
 """
 import argparse
 import json
@@ -26,20 +28,13 @@ from oracle.contraction import evaluate_circuit  # noqa: E402
 from qflex_amd import circuits  # noqa: E402
 
 
-def restricted_ordering(n_slices):
-    lines = circuits.corner_inward_7x7().splitlines()
+def cut_subsets(n_slices):
+    """Value subsets of the two bond cuts for an n_slices sub-job."""
     if n_slices >= 256:
-        return "\n".join(lines) + "\n"
-    cut_lines = [i for i, l in enumerate(lines) if l.startswith("cut ()") and len(l.split()) == 4]
-    first, second = cut_lines
-    v1 = ",".join(str(v) for v in range(max(1, n_slices // 16)))
-    v2 = ",".join(str(v) for v in range(min(16, n_slices)))
+        return None, None
     if n_slices < 16:
-        lines[first] = lines[first].replace("()", "(0)")
-        lines[second] = lines[second].replace("()", f"({v2})")
-    else:
-        lines[first] = lines[first].replace("()", f"({v1})")
-    return "\n".join(lines) + "\n"
+        return [0], list(range(n_slices))
+    return list(range(n_slices // 16)), None
 
 
 def main():
@@ -48,7 +43,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     circ = circuits.random_circuit(7, 7, 32, args.seed)
-    order = restricted_ordering(args.slices)
+    cut_values = cut_subsets(args.slices)
+    order = circuits.corner_inward_7x7(cut_values)
     grid = circuits.grid_text(7, 7)
     t = time.time()
     r = evaluate_circuit(circ, order, grid, "0" * 49, "", dtype=np.complex128)
@@ -58,6 +54,7 @@ def main():
         "circuit": {"I": 7, "J": 7, "depth": 32, "seed": args.seed, "layout": "8pattern",
                     "gates": "xyt"},
         "slices": args.slices,
+        "cut_values": list(cut_values),
         "ordering": order.splitlines(),
         "cpu_seconds": time.time() - t,
         "amplitudes": [[s, a.real, a.imag] for s, a in r],

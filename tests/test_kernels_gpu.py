@@ -87,7 +87,8 @@ def test_reorder_large_rank28_like(ctx):
 GEMM_SHAPES = [
     (2, 2, 2), (4, 2, 2), (16, 16, 16), (64, 16, 8), (100, 50, 37), (128, 64, 16),
     (256, 256, 256), (4096, 256, 256), (1024, 512, 256), (513, 129, 65), (64, 4096, 16),
-    (1, 64, 4096), (1, 3, 5), (300, 1, 77), (5, 1, 3),
+    (1, 64, 4096), (1, 3, 5), (300, 1, 77), (5, 1, 3), (300, 130, 40), (1000, 200, 24),
+    (4096, 4096, 64), (8, 8, 1 << 20), (3, 5, 5000), (64, 64, 1024), (1024, 1024, 40),
 ]
 TOL = {capi.FP32: 2e-6, capi.FP32_EXACT: 2e-6, capi.SPLIT_BF16X3: 5e-6, capi.SPLIT_BF16: 3e-4}
 
@@ -103,7 +104,11 @@ def test_cgemm_vs_complex128(ctx, m, n, k, prec):
     ctx.cgemm(prec, m, n, k, A, B, C)
     got = ctx.download(C, np.complex64, (m, n))
     err = relerr(got.astype(np.complex128), ref)
-    assert err <= TOL[prec], (m, n, k, prec, err)
+    tol = TOL[prec]
+    if prec == capi.FP32_EXACT:
+        # one sequential fmaf chain per output: rounding error grows ~ sqrt(k)
+        tol = max(tol, 1e-7 * np.sqrt(k))
+    assert err <= tol, (m, n, k, prec, err)
     for buf in (A, B, C):
         buf.free()
 
@@ -121,11 +126,11 @@ def test_cgemm_reference_known_answer(ctx):
 
 def test_cgemm_exact_integer_inputs_bit_exact(ctx):
     rng = np.random.default_rng(11)
-    for (m, n, k) in [(64, 16, 8), (200, 96, 40), (130, 70, 300)]:
+    for (m, n, k) in [(64, 16, 8), (200, 96, 40), (130, 70, 300), (300, 130, 40)]:
         a = (rng.integers(-8, 8, (m, k)) + 1j * rng.integers(-8, 8, (m, k))).astype(np.complex64)
         b = (rng.integers(-8, 8, (k, n)) + 1j * rng.integers(-8, 8, (k, n))).astype(np.complex64)
         ref = (a.astype(np.complex128) @ b.astype(np.complex128)).astype(np.complex64)
-        for prec in (capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16):
+        for prec in (capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16, capi.SPLIT_BF16X3):
             A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
             ctx.cgemm(prec, m, n, k, A, B, C)
             got = ctx.download(C, np.complex64, (m, n))
