@@ -6,9 +6,10 @@ C++ host + HIP kernels): build and flatten the network on the GPU, then
 contract all 256 cut slices of the corner-inward ordering and deliver the
 64 amplitudes of the open 2x3 output region (FP64 sums across slices).
 
-Precision variants of the complex GEMMs: fp32 (FP32 MFMA), split_bf16x3
-(BF16 hi+mid+lo, 6 products, FP32 accumulate) and split_bf16 (hi+mid,
-3 products).  Each is timed; its norm-wise relative error is measured on a
+Precision variants of the complex GEMMs: fp32 (FP32 MFMA), split_fp16
+(operands scaled by powers of two, FP16 hi+lo, 3 products, FP32
+accumulate), split_bf16x3 (BF16 hi+mid+lo, 6 products) and split_bf16
+(hi+mid, 3 products).  Each is timed; its norm-wise relative error is measured on a
 16-slice sub-job against committed complex128 oracle amplitudes.  `value`
 is the fastest variant whose error is <= 1e-5.
 
@@ -26,7 +27,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "amplitudes/s, 7x7 depth-32 grid: plain FP32 vs split-BF16 at rel.err≤1e-5"
-PEAK_TFLOPS = {"fp32": 157.3, "split_bf16x3": 2500.0, "split_bf16": 2500.0}
+PEAK_TFLOPS = {"fp32": 157.3, "split_fp16": 2500.0, "split_bf16x3": 2500.0, "split_bf16": 2500.0}
 HBM_GBPS = 8000.0
 ERR_BAR = 1e-5
 
@@ -135,6 +136,29 @@ def microbench(args):
     rng = np.random.default_rng(0)
     out = []
     for shape in args.microbench.split(";"):
+        if shape.startswith("P:"):
+            # permutation: "P:d0,d1,...:p0,p1,..." (complex64 tensor)
+            _, dims_s, perm_s = shape.split(":")
+            dims = [int(x) for x in dims_s.split(",")]
+            perm = [int(x) for x in perm_s.split(",")]
+            nel = int(np.prod(dims))
+            src, dst = ctx.alloc(nel * 8), ctx.alloc(nel * 8)
+            for _ in range(args.warmup):
+                ctx.reorder(src, dst, dims, perm)
+            ctx.profile(True)
+            ctx.profile_reset()
+            for _ in range(args.steps):
+                ctx.reorder(src, dst, dims, perm)
+            prof = ctx.profile_read()
+            prof = prof["permute"] if prof["permute"]["launches"] else prof["copy"]
+            ctx.profile(False)
+            ms = prof["ms"] / max(prof["launches"], 1)
+            out.append({"permute": dims, "perm": perm, "avg_ms": ms,
+                        "gbps": 16.0 * nel / ms / 1e6, "frac_of_hbm": 16.0 * nel / ms / 1e6 / HBM_GBPS})
+            log(f"[bench] permute {dims} {perm}: {ms:.3f} ms, {out[-1]['gbps']:.0f} GB/s")
+            src.free()
+            dst.free()
+            continue
         m, n, k = (int(x) for x in shape.split(","))
         a = (rng.standard_normal((m, k)) + 1j * rng.standard_normal((m, k))).astype(np.complex64)
         b = (rng.standard_normal((k, n)) + 1j * rng.standard_normal((k, n))).astype(np.complex64)
@@ -170,7 +194,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variants", default="fp32,split_bf16x3,split_bf16")
+    ap.add_argument("--variants", default="fp32,split_fp16,split_bf16x3,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--microbench", default="",
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "

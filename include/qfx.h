@@ -50,7 +50,9 @@ enum qfx_precision {
   QFX_PREC_FP32 = 0,       /* FP32 MFMA, FP32 accumulate */
   QFX_PREC_SPLIT_BF16 = 1, /* A,B = hi+mid BF16; hi*hi + hi*mid + mid*hi, FP32 acc */
   QFX_PREC_FP32_EXACT = 2, /* scalar fmaf in K order (defined reduction order) */
-  QFX_PREC_SPLIT_BF16X3 = 3 /* A,B = hi+mid+lo BF16; 6 cross products, FP32 acc */
+  QFX_PREC_SPLIT_BF16X3 = 3, /* A,B = hi+mid+lo BF16; 6 cross products, FP32 acc */
+  QFX_PREC_SPLIT_FP16 = 4    /* A,B scaled by powers of two to max|x| in [2^14, 2^15),
+                                split into FP16 hi + lo; hi*hi + hi*lo + lo*hi, FP32 acc */
 };
 
 /* Kernel classes reported by qfx_profile_read. */

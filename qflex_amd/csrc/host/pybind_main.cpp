@@ -5,7 +5,8 @@
 // (grid/circuit/ordering as lists of lines or *_filename, initial_state(s),
 // final_state(s), memory_limit), the same single-state restriction, and
 // errors printed to stderr with [] returned.  Two extra optional keys:
-// "precision" ("fp32" | "split_bf16" | "split_bf16x3" | "fp32_exact") and
+// "precision" ("fp32" | "split_fp16" | "split_bf16x3" | "split_bf16" |
+// "fp32_exact") and
 // "verbose" (int).
 //
 // The remaining functions expose the host's Tensor / contraction API to the
@@ -86,8 +87,9 @@ int parse_precision(const std::string& p) {
   if (p == "split_bf16") return QFX_PREC_SPLIT_BF16;
   if (p == "split_bf16x3") return QFX_PREC_SPLIT_BF16X3;
   if (p == "fp32_exact") return QFX_PREC_FP32_EXACT;
+  if (p == "split_fp16") return QFX_PREC_SPLIT_FP16;
   throw ERROR_MSG("Unknown precision '", p,
-                  "' (expected fp32, split_bf16, split_bf16x3 or fp32_exact).");
+                  "' (expected fp32, split_fp16, split_bf16x3, split_bf16 or fp32_exact).");
 }
 
 std::vector<std::pair<std::string, std::complex<double>>> evaluate(const py::dict& options) {

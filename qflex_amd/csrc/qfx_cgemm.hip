@@ -315,6 +315,177 @@ __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __re
       }
 }
 
+// ------------------------------------------------------------ split-FP16
+// max(|re|, |im|) over n complex values, as float bits (order-free: the
+// result is the same whatever the reduction order).  One atomic per block;
+// *out must start at 0.
+__global__ __launch_bounds__(256) void amax_kernel(const float2* __restrict__ x, uint64_t n,
+                                                   unsigned int* __restrict__ out) {
+  __shared__ float red[4];
+  float m = 0.f;
+  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += uint64_t(gridDim.x) * blockDim.x) {
+    float2 v = x[i];
+    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    if (m > 0.f) atomicMax(out, __float_as_uint(m));
+  }
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int n) {
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0u;
+}
+
+// Power of two s with max|x| * s in [2^14, 2^15) (FP16 max is 65504).
+__device__ __forceinline__ float fp16_scale(unsigned int amax_bits) {
+  const float amax = __uint_as_float(amax_bits);
+  if (!(amax > 0.f)) return 1.f;
+  int e;
+  frexpf(amax, &e);  // amax = f * 2^e, f in [0.5, 1)
+  return ldexpf(1.f, 15 - e);
+}
+
+// x = hi + lo with hi = fp16(x) and lo = fp16(x - hi): 22 significant bits
+// while the residual is an FP16 normal, i.e. for |x| >= 2^-3 after scaling
+// (elements down to 2^-18 of the operand's max); smaller elements keep
+// fewer bits, which is below 2^-40 of the max in absolute terms.
+__device__ __forceinline__ void split_fp16(float x, _Float16& h, _Float16& l) {
+  h = static_cast<_Float16>(x);
+  l = static_cast<_Float16>(x - static_cast<float>(h));
+}
+
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+
+// Same tiling as cgemm_bf16_mfma_kernel; two FP16 planes (hi, lo) of the
+// scaled operands, products hi*hi + hi*lo + lo*hi in one FP32 accumulator,
+// unscaled on output.
+__global__ __launch_bounds__(256) void cgemm_fp16x2_mfma_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
+    int N, int K, const unsigned int* __restrict__ amax) {
+  __shared__ __align__(16) _Float16 As[2][kBM * kRowB];
+  __shared__ __align__(16) _Float16 Bs[2][2 * kBN * kRowB];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave >> 1;
+  const int wn = wave & 1;
+  const int m0 = blockIdx.x * kBM;
+  const int n0 = blockIdx.y * kBN;
+  const float sa = fp16_scale(amax[0]);
+  const float sb = fp16_scale(amax[1]);
+  const float unscale = 1.f / (sa * sb);
+
+  float2 ra[8], rb[4];
+  auto load_tile = [&](int kt) {
+    const int k0 = kt * kBK;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      int r = (tid >> 4) + 16 * h;
+      int kk = tid & 15;
+      int gm = m0 + r, gk = k0 + kk;
+      ra[h] = (gm < M && gk < K) ? A[uint64_t(gm) * K + gk] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      int kk = (tid >> 6) + 4 * h;
+      int nn = tid & 63;
+      int gk = k0 + kk, gn = n0 + nn;
+      rb[h] = (gk < K && gn < N) ? B[uint64_t(gk) * N + gn] : make_float2(0.f, 0.f);
+    }
+  };
+  auto store_tile = [&]() {
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+      int r = (tid >> 4) + 16 * h;
+      int kk = tid & 15;
+      _Float16 xh, xl, yh, yl;
+      split_fp16(ra[h].x * sa, xh, xl);
+      split_fp16(ra[h].y * sa, yh, yl);
+      *reinterpret_cast<halfx2*>(&As[0][r * kRowB + 2 * kk]) = halfx2{xh, yh};
+      *reinterpret_cast<halfx2*>(&As[1][r * kRowB + 2 * kk]) = halfx2{xl, yl};
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      int kk = (tid >> 6) + 4 * h;
+      int nn = tid & 63;
+      _Float16 rh, rl, ih, il;
+      split_fp16(rb[h].x * sb, rh, rl);
+      split_fp16(rb[h].y * sb, ih, il);
+      *reinterpret_cast<halfx2*>(&Bs[0][nn * kRowB + 2 * kk]) = halfx2{rh, -ih};
+      *reinterpret_cast<halfx2*>(&Bs[0][(kBN + nn) * kRowB + 2 * kk]) = halfx2{ih, rh};
+      *reinterpret_cast<halfx2*>(&Bs[1][nn * kRowB + 2 * kk]) = halfx2{rl, -il};
+      *reinterpret_cast<halfx2*>(&Bs[1][(kBN + nn) * kRowB + 2 * kk]) = halfx2{il, rl};
+    }
+  };
+
+  const int half = lane >> 5;
+  const int col = lane & 31;
+  const int part = col & 1;
+
+  floatx16 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+  const int nkt = (K + kBK - 1) / kBK;
+  load_tile(0);
+  for (int kt = 0; kt < nkt; ++kt) {
+    __syncthreads();
+    store_tile();
+    __syncthreads();
+    if (kt + 1 < nkt) load_tile(kt + 1);
+#pragma unroll
+    for (int ks = 0; ks < kBK; ks += 8) {
+      const int off = 2 * ks + 8 * half;
+      halfx8 a[2][2], b[2][2];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+          a[t][i] = *reinterpret_cast<const halfx8*>(&As[t][(wm * 64 + i * 32 + col) * kRowB + off]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int rc = wn * 64 + j * 32 + col;
+          const int row = (rc & 1) * kBN + (rc >> 1);
+          b[t][j] = *reinterpret_cast<const halfx8*>(&Bs[t][row * kRowB + off]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+
+  float* Cf = reinterpret_cast<float*>(C);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+        int cn = n0 + wn * 32 + j * 16 + (col >> 1);
+        if (row < M && cn < N)
+          Cf[(uint64_t(row) * N + cn) * 2 + part] = acc[i][j][r] * unscale;
+      }
+}
+
 // ------------------------------------------------------------ split-BF16, large tiles
 // For m >= 256, n >= 128 (k, n even).  512 threads (8 waves as 4 x 2), block
 // tile 256 complex rows x 128 complex columns, each wave 64 rows x 64
@@ -710,7 +881,7 @@ extern "C" int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, con
 extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n,
                              uint64_t k, const qfx_c64* A, const qfx_c64* B, qfx_c64* C) {
   QFX_CHECK_CTX(ctx);
-  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_SPLIT_BF16X3)
+  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_SPLIT_FP16)
     return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: unknown precision");
   if (m == 0 || n == 0) return QFX_OK;
   if (!C) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: null C");
@@ -775,6 +946,22 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     else
       cgemm_bf16_big_kernel<6><<<g2, 512, bf16_big_smem<6>(), ctx->stream>>>(A2, B2, C2, int(m),
                                                                             int(n), int(k));
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
+  if (precision == QFX_PREC_SPLIT_FP16) {
+    // operand scales: max|A| and max|B| on the device (no host round trip)
+    void* ws = nullptr;
+    int st = qfx_workspace(ctx, 16, &ws);
+    if (st != QFX_OK) return st;
+    unsigned int* amax = static_cast<unsigned int*>(ws);
+    zero_u32_kernel<<<1, 64, 0, ctx->stream>>>(amax, 2);
+    amax_kernel<<<cap_grid(std::min<uint64_t>((m * k + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
+        A2, m * k, amax);
+    amax_kernel<<<cap_grid(std::min<uint64_t>((k * n + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
+        B2, k * n, amax + 1);
+    cgemm_fp16x2_mfma_kernel<<<grid, 256, 0, ctx->stream>>>(A2, B2, C2, int(m), int(n), int(k),
+                                                            amax);
     QFX_HIP(hipGetLastError());
     return QFX_OK;
   }

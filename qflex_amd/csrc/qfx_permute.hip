@@ -99,6 +99,70 @@ __global__ __launch_bounds__(BLOCK) void permute_bits_kernel(const float2* __res
   }
 }
 
+// Pair variant: each thread moves element pairs, 16 bytes per global access.
+// Input bit 0 is tile bit 0, so tile elements e, e+1 (input order) are
+// adjacent in memory; output bit 0 is the lowest output bit of the tile, so
+// output-order positions o, o+1 are adjacent too.  Needs TBITS >= 2.
+template <int TBITS, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void permute_pairs_kernel(const float2* __restrict__ src,
+                                                              float2* __restrict__ dst,
+                                                              BitPermParams p) {
+  constexpr int TILE = 1 << TBITS;
+  constexpr int PER = TILE / (2 * BLOCK);  // pairs per thread
+  static_assert(PER >= 1, "tile smaller than two elements per thread");
+  __shared__ float2 lds[TILE + TILE / 32];
+
+  const uint32_t tid = threadIdx.x;
+  // Pair e2 = tid + j*BLOCK covers tile elements 2*e2, 2*e2 + 1.
+  uint64_t in_off[PER], out_off[PER];
+  uint32_t slot0[PER], slot1[PER], rslot[PER];
+  const uint32_t bit0_rank = 1u << p.trank[0];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const uint32_t e = 2u * (tid + uint32_t(j) * BLOCK);
+    uint64_t a = 0, o = 0;
+    uint32_t s = 0;
+    for (int k = 1; k < TBITS; ++k) {
+      if ((e >> k) & 1u) {
+        a |= uint64_t(1) << p.tin[k];
+        s |= 1u << p.trank[k];
+        o |= uint64_t(1) << p.tout[k];
+      }
+    }
+    in_off[j] = a;
+    out_off[j] = o;
+    slot0[j] = lds_slot(s);
+    slot1[j] = lds_slot(s | bit0_rank);
+    rslot[j] = lds_slot(e);
+  }
+
+  for (uint64_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
+    uint64_t bin = 0, bout = 0;
+    for (int k = 0; k < p.nfree; ++k) {
+      if ((t >> k) & 1u) {
+        bin |= uint64_t(1) << p.fin[k];
+        bout |= uint64_t(1) << p.fout[k];
+      }
+    }
+    float4 v[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) v[j] = *reinterpret_cast<const float4*>(src + bin + in_off[j]);
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      lds[slot0[j]] = make_float2(v[j].x, v[j].y);
+      lds[slot1[j]] = make_float2(v[j].z, v[j].w);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+      const float2 x = lds[rslot[j]];
+      const float2 y = lds[rslot[j] + 1];
+      *reinterpret_cast<float4*>(dst + bout + out_off[j]) = make_float4(x.x, x.y, y.x, y.y);
+    }
+    __syncthreads();
+  }
+}
+
 struct GenericParams {
   int rank;
   uint64_t total;
@@ -129,6 +193,12 @@ template <int TBITS, int BLOCK>
 void launch_bits(const BitPermParams& p, const float2* src, float2* dst, hipStream_t s) {
   uint64_t g = std::min<uint64_t>(p.ntiles, 256 * 16);
   permute_bits_kernel<TBITS, BLOCK><<<unsigned(g), BLOCK, 0, s>>>(src, dst, p);
+}
+
+template <int TBITS, int BLOCK>
+void launch_pairs(const BitPermParams& p, const float2* src, float2* dst, hipStream_t s) {
+  uint64_t g = std::min<uint64_t>(p.ntiles, 256 * 8);
+  permute_pairs_kernel<TBITS, BLOCK><<<unsigned(g), BLOCK, 0, s>>>(src, dst, p);
 }
 
 }  // namespace
@@ -213,9 +283,14 @@ extern "C" int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* ds
       for (int j = 0; j < bits[a]; ++j) ob[in_start[a] + j] = out_start[a] + j;
     for (int i = 0; i < nbits; ++i) ib_of_ob[ob[i]] = i;
 
-    const int tile_target = 10;  // 1024-element tiles
+    // Tiles of 2^12 elements (6 lowest input bits + 6 lowest output bits)
+    // moved as 16-byte pairs for tensors of >= 2^16 elements; 2^10-element
+    // tiles of single elements below that.
+    const bool pairs = nbits >= 16 && (reinterpret_cast<uintptr_t>(src) % 16 == 0) &&
+                       (reinterpret_cast<uintptr_t>(dst) % 16 == 0);
+    const int tile_target = pairs ? 12 : 10;
     int tbits = std::min(nbits, tile_target);
-    int abits = std::min(tbits, 5);
+    int abits = std::min(tbits, pairs ? 6 : 5);
     std::vector<char> in_t(nbits, 0);
     for (int i = 0; i < abits; ++i) in_t[i] = 1;
     int have = abits;
@@ -249,7 +324,9 @@ extern "C" int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* ds
       }
     p.nfree = nf;
     p.ntiles = uint64_t(1) << nf;
-    switch (tbits) {
+    if (pairs) {
+      launch_pairs<12, 256>(p, s2, d2, ctx->stream);
+    } else switch (tbits) {
       case 10: launch_bits<10, 256>(p, s2, d2, ctx->stream); break;
       case 9: launch_bits<9, 256>(p, s2, d2, ctx->stream); break;
       case 8: launch_bits<8, 256>(p, s2, d2, ctx->stream); break;

@@ -158,7 +158,7 @@ def test_simple_initialize_data_known_answer():
 
 def test_simple_circuit_known_answer():
     k = KATS["simple_circuit"]
-    for prec in ("fp32", "fp32_exact", "split_bf16x3"):
+    for prec in ("fp32", "fp32_exact", "split_bf16x3", "split_fp16"):
         r = qflex.evaluate({"circuit": k["circuit"], "ordering": k["ordering"], "grid": k["grid"],
                             "initial_state": k["initial_state"], "final_state": k["final_state"],
                             "precision": prec})
@@ -182,11 +182,13 @@ def run_engine(circ, order, grid, initial, final, precision="fp32"):
                            "memory_limit": 1 << 36})
 
 
-TOL = {"fp32": 1e-5, "fp32_exact": 1e-5, "split_bf16x3": 1e-5, "split_bf16": 3e-3}
+TOL = {"fp32": 1e-5, "fp32_exact": 1e-5, "split_bf16x3": 1e-5, "split_bf16": 3e-3,
+       "split_fp16": 1e-5}
 
 
 @pytest.mark.parametrize("I,J,depth,seed", [(3, 3, 8, 1), (4, 4, 16, 2), (5, 4, 16, 3)])
-@pytest.mark.parametrize("precision", ["fp32", "fp32_exact", "split_bf16x3", "split_bf16"])
+@pytest.mark.parametrize("precision", ["fp32", "fp32_exact", "split_bf16x3", "split_bf16",
+                                       "split_fp16"])
 def test_snake_vs_state_vector(I, J, depth, seed, precision):
     circ = circuits.random_circuit(I, J, depth, seed)
     psi = statevector.final_state_vector(Circuit(circ), I, J, [], "0" * (I * J))

@@ -90,11 +90,13 @@ GEMM_SHAPES = [
     (1, 64, 4096), (1, 3, 5), (300, 1, 77), (5, 1, 3), (300, 130, 40), (1000, 200, 24),
     (4096, 4096, 64), (8, 8, 1 << 20), (3, 5, 5000), (64, 64, 1024), (1024, 1024, 40),
 ]
-TOL = {capi.FP32: 2e-6, capi.FP32_EXACT: 2e-6, capi.SPLIT_BF16X3: 5e-6, capi.SPLIT_BF16: 3e-4}
+TOL = {capi.FP32: 2e-6, capi.FP32_EXACT: 2e-6, capi.SPLIT_BF16X3: 5e-6, capi.SPLIT_BF16: 3e-4,
+       capi.SPLIT_FP16: 5e-6}
 
 
 @pytest.mark.parametrize("m,n,k", GEMM_SHAPES)
-@pytest.mark.parametrize("prec", [capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16, capi.SPLIT_BF16X3])
+@pytest.mark.parametrize("prec", [capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16, capi.SPLIT_BF16X3,
+                                  capi.SPLIT_FP16])
 def test_cgemm_vs_complex128(ctx, m, n, k, prec):
     rng = np.random.default_rng(m * 7 + n * 3 + k)
     a = rand_c64(rng, (m, k))
@@ -130,7 +132,8 @@ def test_cgemm_exact_integer_inputs_bit_exact(ctx):
         a = (rng.integers(-8, 8, (m, k)) + 1j * rng.integers(-8, 8, (m, k))).astype(np.complex64)
         b = (rng.integers(-8, 8, (k, n)) + 1j * rng.integers(-8, 8, (k, n))).astype(np.complex64)
         ref = (a.astype(np.complex128) @ b.astype(np.complex128)).astype(np.complex64)
-        for prec in (capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16, capi.SPLIT_BF16X3):
+        for prec in (capi.FP32, capi.FP32_EXACT, capi.SPLIT_BF16, capi.SPLIT_BF16X3,
+                     capi.SPLIT_FP16):
             A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
             ctx.cgemm(prec, m, n, k, A, B, C)
             got = ctx.download(C, np.complex64, (m, n))
