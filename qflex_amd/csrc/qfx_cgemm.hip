@@ -486,27 +486,78 @@ __global__ __launch_bounds__(256) void cgemm_fp16x2_mfma_kernel(
       }
 }
 
-// ------------------------------------------------------------ split-BF16, large tiles
-// For m >= 256, n >= 128 (k, n even).  512 threads (8 waves as 4 x 2), block
-// tile 256 complex rows x 128 complex columns, each wave 64 rows x 64
-// complex columns = 2 x 4 MFMA tiles of 32 x 32 (128 accumulator registers).
-// One stage = 8 complex k = 16 reals = one v_mfma_f32_32x32x16_bf16 k-step.
-// LDS holds two stages of NT BF16 planes:
+// ------------------------------------------------------------ split forms, large tiles
+// For m, n >= 1024 (k, n even).  512 threads (8 waves as 4 x 2), block tile
+// 256 complex rows x 128 complex columns, each wave 64 rows x 64 complex
+// columns = 2 x 4 MFMA tiles of 32 x 32 (128 accumulator registers).
+// One stage = 8 complex k = 16 reals = one 32x32x16 MFMA k-step.
+// LDS holds two stages of NT low-precision planes:
 //   As[buf][t][row][16 reals + 8 pad]      (48-byte rows: conflict-free b128)
 //   Bs[buf][t][part*128 + n][16 + 8 pad]   Bhat columns, as in the kernel above
 // The next stage's global loads are issued before the MFMAs of the current
 // one; its split and LDS stores follow the MFMAs; one barrier per stage.
+// MODE 0: BF16 hi+mid+lo, 6 products; 1: BF16 hi+mid, 3 products;
+// 2: FP16 hi+lo of operands scaled by powers of two (amax), 3 products.
 constexpr int kBBM = 256, kBBN = 128, kBBK = 8;
-constexpr int kRowL = 24;  // BF16 per LDS row
+constexpr int kRowL = 24;  // 16-bit elements per LDS row
 
-template <int TERMS>
-__global__ __launch_bounds__(512) void cgemm_bf16_big_kernel(const float2* __restrict__ A,
-                                                             const float2* __restrict__ B,
-                                                             float2* __restrict__ C, int M,
-                                                             int N, int K) {
-  constexpr int NT = TERMS >= 6 ? 3 : 2;
-  extern __shared__ __align__(16) __bf16 smem[];
-  // [buf][plane][rows][kRowL]: A planes then B planes per buffer
+template <int MODE>
+struct SplitTraits;
+template <>
+struct SplitTraits<0> {
+  typedef __bf16 T;
+  typedef bf16x8 V;
+  static constexpr int NT = 3;
+  static constexpr int PRODUCTS = 6;
+};
+template <>
+struct SplitTraits<1> {
+  typedef __bf16 T;
+  typedef bf16x8 V;
+  static constexpr int NT = 2;
+  static constexpr int PRODUCTS = 3;
+};
+template <>
+struct SplitTraits<2> {
+  typedef _Float16 T;
+  typedef halfx8 V;
+  static constexpr int NT = 2;
+  static constexpr int PRODUCTS = 3;
+};
+
+template <int MODE>
+__device__ __forceinline__ void split_terms(float x, typename SplitTraits<MODE>::T* t) {
+  if constexpr (MODE == 2) {
+    split_fp16(x, t[0], t[1]);
+  } else {
+    __bf16 h, m, l;
+    split3(x, h, m, l);
+    t[0] = h;
+    t[1] = m;
+    if constexpr (SplitTraits<MODE>::NT == 3) t[2] = l;
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
+                                           typename SplitTraits<MODE>::V b, floatx16 c) {
+  if constexpr (MODE == 2)
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  else
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __restrict__ A,
+                                                              const float2* __restrict__ B,
+                                                              float2* __restrict__ C, int M,
+                                                              int N, int K,
+                                                              const unsigned int* __restrict__ amax) {
+  typedef typename SplitTraits<MODE>::T T;
+  typedef typename SplitTraits<MODE>::V V;
+  constexpr int NT = SplitTraits<MODE>::NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* smem = reinterpret_cast<T*>(smem_raw);
   constexpr int kAPlane = kBBM * kRowL;
   constexpr int kBPlane = 2 * kBBN * kRowL;
   constexpr int kBuf = NT * (kAPlane + kBPlane);
@@ -518,6 +569,11 @@ __global__ __launch_bounds__(512) void cgemm_bf16_big_kernel(const float2* __res
   const int wn = wave & 1;   // 0..1: 64 complex columns each
   const int m0 = blockIdx.x * kBBM;
   const int n0 = blockIdx.y * kBBN;
+  float sa = 1.f, sb = 1.f;
+  if constexpr (MODE == 2) {
+    sa = fp16_scale(amax[0]);
+    sb = fp16_scale(amax[1]);
+  }
 
   // global -> register assignment
   // A: 256 rows x 8 k = 1024 float4 (2 complex each); f = tid + 512*h:
@@ -543,41 +599,41 @@ __global__ __launch_bounds__(512) void cgemm_bf16_big_kernel(const float2* __res
                               : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   };
-  typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  typedef T T2 __attribute__((ext_vector_type(2)));
+  typedef T T4 __attribute__((ext_vector_type(4)));
   auto store_tile = [&](int buf) {
-    __bf16* base = smem + buf * kBuf;
+    T* base = smem + buf * kBuf;
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int f = tid + 512 * h;
       const int row = f >> 2, q = f & 3;
-      float v[4] = {ra[h].x, ra[h].y, ra[h].z, ra[h].w};
-      __bf16 t3[3][4];
+      float v[4] = {ra[h].x * sa, ra[h].y * sa, ra[h].z * sa, ra[h].w * sa};
+      T t3[4][3];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) split3(v[e], t3[0][e], t3[1][e], t3[2][e]);
+      for (int e = 0; e < 4; ++e) split_terms<MODE>(v[e], t3[e]);
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
-        bf16x4 w = {t3[t][0], t3[t][1], t3[t][2], t3[t][3]};
-        *reinterpret_cast<bf16x4*>(base + t * kAPlane + row * kRowL + 4 * q) = w;
+        T4 w = {t3[0][t], t3[1][t], t3[2][t], t3[3][t]};
+        *reinterpret_cast<T4*>(base + t * kAPlane + row * kRowL + 4 * q) = w;
       }
     }
     {
       const int kk = tid & 7, nq = tid >> 3;
-      __bf16* bb = base + NT * kAPlane;
-      float v[4] = {rb.x, rb.y, rb.z, rb.w};  // (Br, Bi) of n, then of n+1
-      __bf16 t3[3][4];
+      T* bb = base + NT * kAPlane;
+      float v[4] = {rb.x * sb, rb.y * sb, rb.z * sb, rb.w * sb};  // (Br, Bi) of n, then n+1
+      T t3[4][3];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) split3(v[e], t3[0][e], t3[1][e], t3[2][e]);
+      for (int e = 0; e < 4; ++e) split_terms<MODE>(v[e], t3[e]);
 #pragma unroll
       for (int t = 0; t < NT; ++t) {
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
           const int n = 2 * nq + c;
-          const __bf16 br = t3[t][2 * c], bi = t3[t][2 * c + 1];
-          bf16x2 p0 = {br, static_cast<__bf16>(-static_cast<float>(bi))};
-          bf16x2 p1 = {bi, br};
-          *reinterpret_cast<bf16x2*>(bb + t * kBPlane + n * kRowL + 2 * kk) = p0;
-          *reinterpret_cast<bf16x2*>(bb + t * kBPlane + (kBBN + n) * kRowL + 2 * kk) = p1;
+          const T br = t3[2 * c][t], bi = t3[2 * c + 1][t];
+          T2 p0 = {br, static_cast<T>(-static_cast<float>(bi))};
+          T2 p1 = {bi, br};
+          *reinterpret_cast<T2*>(bb + t * kBPlane + n * kRowL + 2 * kk) = p0;
+          *reinterpret_cast<T2*>(bb + t * kBPlane + (kBBN + n) * kRowL + 2 * kk) = p1;
         }
       }
     }
@@ -602,39 +658,40 @@ __global__ __launch_bounds__(512) void cgemm_bf16_big_kernel(const float2* __res
   for (int kt = 0; kt < nkt; ++kt) {
     const int buf = kt & 1;
     if (kt + 1 < nkt) load_tile(kt + 1);
-    const __bf16* base = smem + buf * kBuf;
-    const __bf16* bb = base + NT * kAPlane;
-    bf16x8 a[NT][2], b[NT][4];
+    const T* base = smem + buf * kBuf;
+    const T* bb = base + NT * kAPlane;
+    V a[NT][2], b[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
 #pragma unroll
       for (int i = 0; i < 2; ++i)
-        a[t][i] = *reinterpret_cast<const bf16x8*>(base + t * kAPlane +
-                                                   (wm * 64 + i * 32 + col) * kRowL + 8 * half);
+        a[t][i] = *reinterpret_cast<const V*>(base + t * kAPlane +
+                                              (wm * 64 + i * 32 + col) * kRowL + 8 * half);
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int rc = wn * 128 + j * 32 + col;  // real column of Bhat in the block
         const int row = (rc & 1) * kBBN + (rc >> 1);
-        b[t][j] = *reinterpret_cast<const bf16x8*>(bb + t * kBPlane + row * kRowL + 8 * half);
+        b[t][j] = *reinterpret_cast<const V*>(bb + t * kBPlane + row * kRowL + 8 * half);
       }
     }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
-        if (TERMS >= 6) {
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2][i], b[0][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[2][j], acc[i][j], 0, 0, 0);
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[1][j], acc[i][j], 0, 0, 0);
+        if constexpr (SplitTraits<MODE>::PRODUCTS == 6) {
+          acc[i][j] = mfma16<MODE>(a[2][i], b[0][j], acc[i][j]);
+          acc[i][j] = mfma16<MODE>(a[0][i], b[2][j], acc[i][j]);
+          acc[i][j] = mfma16<MODE>(a[1][i], b[1][j], acc[i][j]);
         }
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1][i], b[0][j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[1][j], acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0][i], b[0][j], acc[i][j], 0, 0, 0);
+        acc[i][j] = mfma16<MODE>(a[1][i], b[0][j], acc[i][j]);
+        acc[i][j] = mfma16<MODE>(a[0][i], b[1][j], acc[i][j]);
+        acc[i][j] = mfma16<MODE>(a[0][i], b[0][j], acc[i][j]);
       }
     if (kt + 1 < nkt) store_tile(buf ^ 1);
     __syncthreads();
   }
 
+  const float unscale = 1.f / (sa * sb);
   float* Cf = reinterpret_cast<float*>(C);
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -644,22 +701,21 @@ __global__ __launch_bounds__(512) void cgemm_bf16_big_kernel(const float2* __res
       for (int r = 0; r < 16; ++r) {
         const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
         const int cn = n0 + wn * 64 + j * 16 + (col >> 1);
-        if (row < M && cn < N) Cf[(uint64_t(row) * N + cn) * 2 + part] = acc[i][j][r];
+        if (row < M && cn < N) Cf[(uint64_t(row) * N + cn) * 2 + part] = acc[i][j][r] * unscale;
       }
 }
 
-template <int TERMS>
-size_t bf16_big_smem() {
-  constexpr int NT = TERMS >= 6 ? 3 : 2;
-  return size_t(2) * NT * (kBBM * kRowL + 2 * kBBN * kRowL) * sizeof(__bf16);
+template <int MODE>
+size_t split_big_smem() {
+  return size_t(2) * SplitTraits<MODE>::NT * (kBBM * kRowL + 2 * kBBN * kRowL) * 2;
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel once per process.
-template <int TERMS>
-hipError_t bf16_big_prepare() {
+template <int MODE>
+hipError_t split_big_prepare() {
   static hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&cgemm_bf16_big_kernel<TERMS>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, int(bf16_big_smem<TERMS>()));
+      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, int(split_big_smem<MODE>()));
   return e;
 }
 
@@ -936,30 +992,39 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   const bool big_tile = m >= 1024 && n >= 1024 && (k % 2 == 0) && (n % 2 == 0) &&
                         (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
                         (reinterpret_cast<uintptr_t>(B) % 16 == 0);
-  if (precision != QFX_PREC_FP32 && big_tile) {
-    dim3 g2{unsigned((m + kBBM - 1) / kBBM), unsigned((n + kBBN - 1) / kBBN), 1};
-    if (g2.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
-    QFX_HIP(precision == QFX_PREC_SPLIT_BF16 ? bf16_big_prepare<3>() : bf16_big_prepare<6>());
-    if (precision == QFX_PREC_SPLIT_BF16)
-      cgemm_bf16_big_kernel<3><<<g2, 512, bf16_big_smem<3>(), ctx->stream>>>(A2, B2, C2, int(m),
-                                                                            int(n), int(k));
-    else
-      cgemm_bf16_big_kernel<6><<<g2, 512, bf16_big_smem<6>(), ctx->stream>>>(A2, B2, C2, int(m),
-                                                                            int(n), int(k));
-    QFX_HIP(hipGetLastError());
-    return QFX_OK;
-  }
+  unsigned int* amax = nullptr;
   if (precision == QFX_PREC_SPLIT_FP16) {
     // operand scales: max|A| and max|B| on the device (no host round trip)
     void* ws = nullptr;
     int st = qfx_workspace(ctx, 16, &ws);
     if (st != QFX_OK) return st;
-    unsigned int* amax = static_cast<unsigned int*>(ws);
+    amax = static_cast<unsigned int*>(ws);
     zero_u32_kernel<<<1, 64, 0, ctx->stream>>>(amax, 2);
     amax_kernel<<<cap_grid(std::min<uint64_t>((m * k + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
         A2, m * k, amax);
     amax_kernel<<<cap_grid(std::min<uint64_t>((k * n + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
         B2, k * n, amax + 1);
+  }
+  if (precision != QFX_PREC_FP32 && big_tile) {
+    dim3 g2{unsigned((m + kBBM - 1) / kBBM), unsigned((n + kBBN - 1) / kBBN), 1};
+    if (g2.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
+    if (precision == QFX_PREC_SPLIT_BF16X3) {
+      QFX_HIP(split_big_prepare<0>());
+      cgemm_split_big_kernel<0><<<g2, 512, split_big_smem<0>(), ctx->stream>>>(
+          A2, B2, C2, int(m), int(n), int(k), amax);
+    } else if (precision == QFX_PREC_SPLIT_BF16) {
+      QFX_HIP(split_big_prepare<1>());
+      cgemm_split_big_kernel<1><<<g2, 512, split_big_smem<1>(), ctx->stream>>>(
+          A2, B2, C2, int(m), int(n), int(k), amax);
+    } else {
+      QFX_HIP(split_big_prepare<2>());
+      cgemm_split_big_kernel<2><<<g2, 512, split_big_smem<2>(), ctx->stream>>>(
+          A2, B2, C2, int(m), int(n), int(k), amax);
+    }
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
+  if (precision == QFX_PREC_SPLIT_FP16) {
     cgemm_fp16x2_mfma_kernel<<<grid, 256, 0, ctx->stream>>>(A2, B2, C2, int(m), int(n), int(k),
                                                             amax);
     QFX_HIP(hipGetLastError());
