@@ -369,6 +369,10 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
     stats_.reorders++;
     stats_.reorder_elements += double(B.size());
   }
+  if (global::verbose > 1)
+    std::cerr << "multiply m=" << left_dim << " n=" << right_dim << " k=" << common_dim
+              << " reorder_a=" << (pa != A.ptr ? A.size() : 0)
+              << " reorder_b=" << (pb != B.ptr ? B.size() : 0) << std::endl;
   device_multiply(pa, pb, out, left_dim, right_dim, common_dim, !left.empty(), !right.empty());
   stats_.multiplies++;
   stats_.flops += 8.0 * double(left_dim) * double(right_dim) * double(common_dim);

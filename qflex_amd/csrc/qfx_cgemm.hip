@@ -487,18 +487,24 @@ __global__ __launch_bounds__(256) void cgemm_fp16x2_mfma_kernel(
 }
 
 // ------------------------------------------------------------ split forms, large tiles
-// For m, n >= 1024 (k, n even).  512 threads (8 waves as 4 x 2), block tile
-// 256 complex rows x 128 complex columns, each wave 64 rows x 64 complex
-// columns = 2 x 4 MFMA tiles of 32 x 32 (128 accumulator registers).
-// One stage = 8 complex k = 16 reals = one 32x32x16 MFMA k-step.
-// LDS holds two stages of NT low-precision planes:
+// For large m and n >= 128 (k, n even).  512 threads = 8 waves as WM x WN
+// (4 x 2 or 2 x 4); each wave owns 64 rows x 64 complex columns = 2 x 4 MFMA
+// tiles of 32 x 32 (128 accumulator registers), so the block tile is
+// 64*WM complex rows x 64*WN complex columns.  One stage = 8 complex k = 16
+// reals = one 32x32x16 MFMA k-step.  LDS holds two stages of NT planes:
 //   As[buf][t][row][16 reals + 8 pad]      (48-byte rows: conflict-free b128)
-//   Bs[buf][t][part*128 + n][16 + 8 pad]   Bhat columns, as in the kernel above
-// The next stage's global loads are issued before the MFMAs of the current
-// one; its split and LDS stores follow the MFMAs; one barrier per stage.
+//   Bs[buf][t][part*BN + n][16 + 8 pad]    Bhat columns, as in the kernel above
+// Software pipeline, one barrier per stage: while the MFMAs of stage s run
+// from LDS buffer s&1, the raw operands of stage s+1 (loaded into registers
+// during stage s-1) are split and stored into the other buffer, and the
+// global loads of stage s+2 are in flight into the register set just freed.
+// The loop body is unrolled by two (static register sets) and branch-free
+// (the last stages reload the final tile into a buffer that is not read
+// again), so the scheduler can interleave MFMA, VALU and LDS instructions;
+// sched_group_barrier fixes that interleave.
 // MODE 0: BF16 hi+mid+lo, 6 products; 1: BF16 hi+mid, 3 products;
 // 2: FP16 hi+lo of operands scaled by powers of two (amax), 3 products.
-constexpr int kBBM = 256, kBBN = 128, kBBK = 8;
+constexpr int kBBK = 8;
 constexpr int kRowL = 24;  // 16-bit elements per LDS row
 
 template <int MODE>
@@ -547,28 +553,32 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-template <int MODE>
+template <int MODE, int WM, int WN>
 __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __restrict__ A,
                                                               const float2* __restrict__ B,
                                                               float2* __restrict__ C, int M,
                                                               int N, int K,
                                                               const unsigned int* __restrict__ amax) {
+  static_assert(WM * WN == 8, "8 waves");
   typedef typename SplitTraits<MODE>::T T;
   typedef typename SplitTraits<MODE>::V V;
   constexpr int NT = SplitTraits<MODE>::NT;
+  constexpr int kBM = 64 * WM, kBN = 64 * WN;
+  constexpr int kAPlane = kBM * kRowL;
+  constexpr int kBPlane = 2 * kBN * kRowL;
+  constexpr int kBuf = NT * (kAPlane + kBPlane);
+  constexpr int kAH = kBM * 4 / 512;  // float4 (2 complex) A loads per thread and stage
+  constexpr int kBH = kBN * 4 / 512;  // float4 B loads per thread and stage
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
-  constexpr int kAPlane = kBBM * kRowL;
-  constexpr int kBPlane = 2 * kBBN * kRowL;
-  constexpr int kBuf = NT * (kAPlane + kBPlane);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave >> 1;  // 0..3: 64 rows each
-  const int wn = wave & 1;   // 0..1: 64 complex columns each
-  const int m0 = blockIdx.x * kBBM;
-  const int n0 = blockIdx.y * kBBN;
+  const int wm = wave / WN;  // 64 rows each
+  const int wn = wave % WN;  // 64 complex columns each
+  const int m0 = blockIdx.x * kBM;
+  const int n0 = blockIdx.y * kBN;
   float sa = 1.f, sb = 1.f;
   if constexpr (MODE == 2) {
     sa = fp16_scale(amax[0]);
@@ -576,35 +586,36 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
   }
 
   // global -> register assignment
-  // A: 256 rows x 8 k = 1024 float4 (2 complex each); f = tid + 512*h:
+  // A: kBM rows x 8 k = 4*kBM float4 (2 complex each); f = tid + 512*h:
   //    row = f >> 2, q = f & 3 (complex k = 2q, 2q+1)
-  // B: 8 k x 128 n = 512 float4; lanes with consecutive tid take
-  //    consecutive k so the LDS stores are conflict-free: k = tid & 7,
-  //    nq = tid >> 3 (complex n = 2nq, 2nq+1)
-  float4 ra[2], rb;
-  auto load_tile = [&](int kt) {
+  // B: 8 k x kBN n = 4*kBN float4; g = tid + 512*h: k = g & 7 (consecutive
+  //    lanes take consecutive k, so the LDS stores are conflict-free),
+  //    nq = g >> 3 (complex n = 2nq, 2nq+1)
+  auto load_tile = [&](int kt, float4* ra, float4* rb) {
     const int k0 = kt * kBBK;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kAH; ++h) {
       const int f = tid + 512 * h;
       const int row = f >> 2, q = f & 3;
       const int gm = m0 + row, gk = k0 + 2 * q;
       ra[h] = (gm < M && gk < K) ? *reinterpret_cast<const float4*>(A + uint64_t(gm) * K + gk)
                                  : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    {
-      const int kk = tid & 7, nq = tid >> 3;
+#pragma unroll
+    for (int h = 0; h < kBH; ++h) {
+      const int g = tid + 512 * h;
+      const int kk = g & 7, nq = g >> 3;
       const int gk = k0 + kk, gn = n0 + 2 * nq;
-      rb = (gk < K && gn < N) ? *reinterpret_cast<const float4*>(B + uint64_t(gk) * N + gn)
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
+      rb[h] = (gk < K && gn < N) ? *reinterpret_cast<const float4*>(B + uint64_t(gk) * N + gn)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   };
   typedef T T2 __attribute__((ext_vector_type(2)));
   typedef T T4 __attribute__((ext_vector_type(4)));
-  auto store_tile = [&](int buf) {
+  auto store_tile = [&](int buf, const float4* ra, const float4* rb) {
     T* base = smem + buf * kBuf;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < kAH; ++h) {
       const int f = tid + 512 * h;
       const int row = f >> 2, q = f & 3;
       float v[4] = {ra[h].x * sa, ra[h].y * sa, ra[h].z * sa, ra[h].w * sa};
@@ -617,10 +628,13 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
         *reinterpret_cast<T4*>(base + t * kAPlane + row * kRowL + 4 * q) = w;
       }
     }
-    {
-      const int kk = tid & 7, nq = tid >> 3;
-      T* bb = base + NT * kAPlane;
-      float v[4] = {rb.x * sb, rb.y * sb, rb.z * sb, rb.w * sb};  // (Br, Bi) of n, then n+1
+    T* bb = base + NT * kAPlane;
+#pragma unroll
+    for (int h = 0; h < kBH; ++h) {
+      const int g = tid + 512 * h;
+      const int kk = g & 7, nq = g >> 3;
+      // (Br, Bi) of n, then of n+1
+      float v[4] = {rb[h].x * sb, rb[h].y * sb, rb[h].z * sb, rb[h].w * sb};
       T t3[4][3];
 #pragma unroll
       for (int e = 0; e < 4; ++e) split_terms<MODE>(v[e], t3[e]);
@@ -633,7 +647,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
           T2 p0 = {br, static_cast<T>(-static_cast<float>(bi))};
           T2 p1 = {bi, br};
           *reinterpret_cast<T2*>(bb + t * kBPlane + n * kRowL + 2 * kk) = p0;
-          *reinterpret_cast<T2*>(bb + t * kBPlane + (kBBN + n) * kRowL + 2 * kk) = p1;
+          *reinterpret_cast<T2*>(bb + t * kBPlane + (kBN + n) * kRowL + 2 * kk) = p1;
         }
       }
     }
@@ -651,13 +665,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
-  const int nkt = (K + kBBK - 1) / kBBK;
-  load_tile(0);
-  store_tile(0);
-  __syncthreads();
-  for (int kt = 0; kt < nkt; ++kt) {
-    const int buf = kt & 1;
-    if (kt + 1 < nkt) load_tile(kt + 1);
+  auto compute = [&](int buf) {
     const T* base = smem + buf * kBuf;
     const T* bb = base + NT * kAPlane;
     V a[NT][2], b[NT][4];
@@ -670,7 +678,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int rc = wn * 128 + j * 32 + col;  // real column of Bhat in the block
-        const int row = (rc & 1) * kBBN + (rc >> 1);
+        const int row = (rc & 1) * kBN + (rc >> 1);
         b[t][j] = *reinterpret_cast<const V*>(bb + t * kBPlane + row * kRowL + 8 * half);
       }
     }
@@ -687,8 +695,43 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
         acc[i][j] = mfma16<MODE>(a[0][i], b[1][j], acc[i][j]);
         acc[i][j] = mfma16<MODE>(a[0][i], b[0][j], acc[i][j]);
       }
-    if (kt + 1 < nkt) store_tile(buf ^ 1);
+  };
+  // Interleave of one stage: global loads, fragment reads, then MFMAs in
+  // pairs around groups of VALU (operand split) and one LDS store.
+  auto schedule = [&]() {
+    constexpr int kMfma = 8 * SplitTraits<MODE>::PRODUCTS;
+    __builtin_amdgcn_sched_group_barrier(0x020, kAH + kBH, 0);  // VMEM read
+    __builtin_amdgcn_sched_group_barrier(0x100, 6 * NT, 0);     // DS read
+#pragma unroll
+    for (int i = 0; i < kMfma / 2; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+      __builtin_amdgcn_sched_group_barrier(0x002, 6, 0);  // VALU
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // DS write
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    }
+  };
+
+  const int nkt = (K + kBBK - 1) / kBBK;
+  float4 ra0[kAH], rb0[kBH], ra1[kAH], rb1[kBH];
+  load_tile(0, ra0, rb0);
+  store_tile(0, ra0, rb0);
+  load_tile(nkt > 1 ? 1 : 0, ra1, rb1);
+  __syncthreads();
+  for (int kt = 0; kt < nkt; kt += 2) {
+    // buffer 0 holds stage kt, registers 1 hold stage kt+1
+    load_tile(min(kt + 2, nkt - 1), ra0, rb0);
+    compute(0);
+    store_tile(1, ra1, rb1);
+    schedule();
     __syncthreads();
+    if (kt + 1 < nkt) {
+      // buffer 1 holds stage kt+1, registers 0 hold stage kt+2
+      load_tile(min(kt + 3, nkt - 1), ra1, rb1);
+      compute(1);
+      store_tile(0, ra0, rb0);
+      schedule();
+      __syncthreads();
+    }
   }
 
   const float unscale = 1.f / (sa * sb);
@@ -705,18 +748,29 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
       }
 }
 
-template <int MODE>
+template <int MODE, int WM, int WN>
 size_t split_big_smem() {
-  return size_t(2) * SplitTraits<MODE>::NT * (kBBM * kRowL + 2 * kBBN * kRowL) * 2;
+  return size_t(2) * SplitTraits<MODE>::NT * (64 * WM * kRowL + 2 * 64 * WN * kRowL) * 2;
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel once per process.
-template <int MODE>
+template <int MODE, int WM, int WN>
 hipError_t split_big_prepare() {
   static hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, int(split_big_smem<MODE>()));
+      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN>),
+      hipFuncAttributeMaxDynamicSharedMemorySize, int(split_big_smem<MODE, WM, WN>()));
   return e;
+}
+
+template <int MODE, int WM, int WN>
+hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
+                            uint64_t m, uint64_t n, uint64_t k, const unsigned int* amax) {
+  hipError_t e = split_big_prepare<MODE, WM, WN>();
+  if (e != hipSuccess) return e;
+  dim3 g{unsigned((m + 64 * WM - 1) / (64 * WM)), unsigned((n + 64 * WN - 1) / (64 * WN)), 1};
+  cgemm_split_big_kernel<MODE, WM, WN><<<g, 512, split_big_smem<MODE, WM, WN>(), stream>>>(
+      A, B, C, int(m), int(n), int(k), amax);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------ GEMV / dot
@@ -875,6 +929,119 @@ __global__ __launch_bounds__(256) void cgemm_splitk_partial_kernel(
   }
 }
 
+// Split-K for at most 8 x 8 outputs and long k (the final merges of a
+// contraction, e.g. 8 x 8 x 2^20).  Each thread keeps all 64 outputs in
+// registers over its k values (A rows and the B row of one k are
+// contiguous loads across lanes), then a butterfly reduce-scatter leaves
+// output o = i*8 + j on lane o of each wave; the 4 waves are summed in LDS
+// in fixed order and sum_partials_kernel adds the blocks' partials in
+// fixed order, so results are deterministic.
+constexpr int kTinyChunk = 1024;  // k values per block
+__global__ __launch_bounds__(256) void cgemm_tiny_mn_partial_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ part,
+    int M, int N, uint64_t K) {
+  __shared__ float2 red[4][64];
+  float2 acc[64];
+#pragma unroll
+  for (int o = 0; o < 64; ++o) acc[o] = make_float2(0.f, 0.f);
+  const uint64_t k0 = uint64_t(blockIdx.x) * kTinyChunk;
+  const uint64_t k1 = k0 + kTinyChunk < K ? k0 + kTinyChunk : K;
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    float2 a[8], b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = i < M ? A[uint64_t(i) * K + k] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = j < N ? B[k * N + j] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i * 8 + j] = cfma(a[i], b[j], acc[i * 8 + j]);
+  }
+  // reduce-scatter: after the step with distance s, a lane keeps the half
+  // of its remaining outputs selected by (lane & s).
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 32, half = 32; s > 0; s >>= 1, half >>= 1) {
+    const bool upper = (lane & s) != 0;
+#pragma unroll
+    for (int o = 0; o < half; ++o) {
+      float2 send = upper ? acc[o] : acc[o + half];
+      float2 keep = upper ? acc[o + half] : acc[o];
+      keep.x += __shfl_xor(send.x, s);
+      keep.y += __shfl_xor(send.y, s);
+      acc[o] = keep;
+    }
+  }
+  red[threadIdx.x >> 6][lane] = acc[0];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    float2 t = red[0][lane];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      t.x += red[w][lane].x;
+      t.y += red[w][lane].y;
+    }
+    const int i = lane >> 3, j = lane & 7;
+    if (i < M && j < N) part[uint64_t(blockIdx.x) * M * N + i * N + j] = t;
+  }
+}
+
+// Small common dimension (k <= 16): exact FP32 on the vector ALUs, one
+// fmaf chain per output in increasing k (the same operation order as
+// cgemm_scalar_kernel, hence bit-identical to it).  Block tile BM x BN
+// (BM = 4096 / BN), each thread 4 rows x 4 columns; A and B tiles staged
+// through LDS with coalesced loads.
+constexpr int kSmallK = 16;
+template <int BN>
+__global__ __launch_bounds__(256) void cgemm_smallk_kernel(const float2* __restrict__ A,
+                                                           const float2* __restrict__ B,
+                                                           float2* __restrict__ C, uint64_t M,
+                                                           int N, int K) {
+  constexpr int TX = BN / 4;        // column groups
+  constexpr int BM = 4 * 256 / TX;  // rows per block
+  __shared__ float2 As[BM][kSmallK + 1];
+  __shared__ float2 Bs[kSmallK][BN];
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  const uint64_t m0 = uint64_t(blockIdx.x) * BM;
+  const int n0 = blockIdx.y * BN;
+  // A tile: rows m0.., all K columns (contiguous in memory)
+  for (int e = threadIdx.x; e < BM * K; e += 256) {
+    const int r = e / K, c = e - r * K;
+    As[r][c] = m0 + r < M ? A[(m0 + r) * K + c] : make_float2(0.f, 0.f);
+  }
+  for (int e = threadIdx.x; e < K * BN; e += 256) {
+    const int r = e / BN, c = e - r * BN;
+    Bs[r][c] = n0 + c < N ? B[uint64_t(r) * N + n0 + c] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  float2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = make_float2(0.f, 0.f);
+  for (int kk = 0; kk < K; ++kk) {
+    float2 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) a[i] = As[ty * 4 + i][kk];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx + TX * j];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = cfma(a[i], b[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint64_t r = m0 + ty * 4 + i;
+    if (r >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = n0 + tx + TX * j;
+      if (c < N) C[r * N + c] = acc[i][j];
+    }
+  }
+}
+
 unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, 65535))); }
 
 }  // namespace
@@ -949,6 +1116,33 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   const double flops = 8.0 * double(m) * double(n) * double(k);
   const double bytes = 8.0 * (double(m) * k + double(k) * n + double(m) * n);
 
+  if (k <= uint64_t(kSmallK) && n <= 65535ULL * 64) {
+    // Small common dimension: exact FP32 on the vector ALUs for every
+    // precision (bit-identical to the defined-order scalar kernel).
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
+    if (n <= 16) {
+      dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
+      cgemm_smallk_kernel<16><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k));
+    } else {
+      dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
+      cgemm_smallk_kernel<64><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k));
+    }
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
+  if (precision != QFX_PREC_FP32_EXACT && m <= 8 && n <= 8 && k >= 1024) {
+    QfxLaunchScope scope(ctx, QFX_K_GEMV, bytes, flops);
+    const uint64_t nchunk = (k + kTinyChunk - 1) / kTinyChunk;
+    void* ws = nullptr;
+    int st = qfx_workspace(ctx, nchunk * m * n * sizeof(float2), &ws);
+    if (st != QFX_OK) return st;
+    float2* part = static_cast<float2*>(ws);
+    cgemm_tiny_mn_partial_kernel<<<unsigned(nchunk), 256, 0, ctx->stream>>>(A2, B2, part, int(m),
+                                                                            int(n), k);
+    sum_partials_kernel<<<1, 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
   const bool big = m >= 64 && n >= 16 && k >= 8 && m <= 0x7fffffffULL &&
                    n <= 0x7fffffffULL && k <= 0x7fffffffULL;
   if (precision == QFX_PREC_FP32_EXACT || !big) {
@@ -989,7 +1183,13 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
   dim3 grid(unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN));
   if (grid.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
-  const bool big_tile = m >= 1024 && n >= 1024 && (k % 2 == 0) && (n % 2 == 0) &&
+  // Large tiles: 128 x 256 (n >= 512) or 256 x 128 complex, one block per
+  // CU, when there are enough of them to fill the chip.  (Three BF16 planes
+  // of the wide tile would exceed the 160 KiB of LDS.)
+  const bool wide = n >= 512 && precision != QFX_PREC_SPLIT_BF16X3;
+  const uint64_t big_blocks = wide ? ((m + 127) / 128) * ((n + 255) / 256)
+                                   : ((m + 255) / 256) * ((n + 127) / 128);
+  const bool big_tile = n >= 128 && big_blocks >= 256 && (k % 2 == 0) && (n % 2 == 0) &&
                         (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
                         (reinterpret_cast<uintptr_t>(B) % 16 == 0);
   unsigned int* amax = nullptr;
@@ -1006,22 +1206,17 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
         B2, k * n, amax + 1);
   }
   if (precision != QFX_PREC_FP32 && big_tile) {
-    dim3 g2{unsigned((m + kBBM - 1) / kBBM), unsigned((n + kBBN - 1) / kBBN), 1};
-    if (g2.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
-    if (precision == QFX_PREC_SPLIT_BF16X3) {
-      QFX_HIP(split_big_prepare<0>());
-      cgemm_split_big_kernel<0><<<g2, 512, split_big_smem<0>(), ctx->stream>>>(
-          A2, B2, C2, int(m), int(n), int(k), amax);
-    } else if (precision == QFX_PREC_SPLIT_BF16) {
-      QFX_HIP(split_big_prepare<1>());
-      cgemm_split_big_kernel<1><<<g2, 512, split_big_smem<1>(), ctx->stream>>>(
-          A2, B2, C2, int(m), int(n), int(k), amax);
-    } else {
-      QFX_HIP(split_big_prepare<2>());
-      cgemm_split_big_kernel<2><<<g2, 512, split_big_smem<2>(), ctx->stream>>>(
-          A2, B2, C2, int(m), int(n), int(k), amax);
-    }
-    QFX_HIP(hipGetLastError());
+    if ((n + 127) / 128 > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
+    hipError_t e;
+    if (precision == QFX_PREC_SPLIT_BF16X3)
+      e = launch_split_big<0, 4, 2>(ctx->stream, A2, B2, C2, m, n, k, amax);
+    else if (precision == QFX_PREC_SPLIT_BF16)
+      e = wide ? launch_split_big<1, 2, 4>(ctx->stream, A2, B2, C2, m, n, k, amax)
+               : launch_split_big<1, 4, 2>(ctx->stream, A2, B2, C2, m, n, k, amax);
+    else
+      e = wide ? launch_split_big<2, 2, 4>(ctx->stream, A2, B2, C2, m, n, k, amax)
+               : launch_split_big<2, 4, 2>(ctx->stream, A2, B2, C2, m, n, k, amax);
+    QFX_HIP(e);
     return QFX_OK;
   }
   if (precision == QFX_PREC_SPLIT_FP16) {

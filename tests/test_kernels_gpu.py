@@ -89,6 +89,12 @@ GEMM_SHAPES = [
     (256, 256, 256), (4096, 256, 256), (1024, 512, 256), (513, 129, 65), (64, 4096, 16),
     (1, 64, 4096), (1, 3, 5), (300, 1, 77), (5, 1, 3), (300, 130, 40), (1000, 200, 24),
     (4096, 4096, 64), (8, 8, 1 << 20), (3, 5, 5000), (64, 64, 1024), (1024, 1024, 40),
+    # small common dimension (vector-ALU tiles of 256x16 and 64x64)
+    (4096, 512, 16), (1000, 20, 16), (300, 7, 3), (70000, 16, 16), (257, 65, 9),
+    # at most 8 x 8 outputs with long k (split-K with register accumulators)
+    (1, 8, 4096), (7, 3, 1500), (8, 1, 2048),
+    # large tiles 256x128 and 128x256 with ragged edges and odd stage counts
+    (32768, 256, 24), (33000, 130, 74), (20000, 600, 36),
 ]
 TOL = {capi.FP32: 2e-6, capi.FP32_EXACT: 2e-6, capi.SPLIT_BF16X3: 5e-6, capi.SPLIT_BF16: 3e-4,
        capi.SPLIT_FP16: 5e-6}
