@@ -115,6 +115,34 @@ int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* dst, int rank
 int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n, uint64_t k,
                   const qfx_c64* A, const qfx_c64* B, qfx_c64* C);
 
+/* Index map of one matrix dimension of a tensor whose dimensions are all
+ * powers of two.  The matrix index i is split into digits, least
+ * significant first: digit j has bits[j] bits and is the tensor axis whose
+ * element stride is 2^shift[j], so that
+ *   offset(i) = sum_j ((i >> (bits[0] + ... + bits[j-1])) & (2^bits[j] - 1)) << shift[j].
+ * ndigits == 0 is the map of a dimension of size 1. */
+typedef struct qfx_index_map {
+  int ndigits; /* 0..16 */
+  unsigned char bits[16];
+  unsigned char shift[16];
+} qfx_index_map;
+
+/* C[m x n] (row-major) = sum_k A[a_rows(m) + a_cols(k)] . B[b_rows(k) + b_cols(n)]:
+ * the contraction of multiply(A, B, C) read straight from the stored index
+ * orders of A and B, replacing the reorders of A to (left, common) and of
+ * B to (common, right) plus _multiply_MM (src/tensor.cpp:1100-1128 and
+ * :960-980).  m, n, k are the products of the maps' digit sizes.
+ * Supported: the split precisions when m, n, k are large enough for the
+ * MFMA tiles, any precision when k <= 16, and the non-exact precisions for
+ * at most 8 x 8 outputs with k >= 1024; the operand read along its
+ * contiguous axis needs digit 0 of that map at stride 1 with at least 3
+ * bits.  Otherwise returns QFX_ERR_UNSUPPORTED without touching C (the
+ * caller then reorders and calls qfx_cgemm_c64). */
+int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c64* A,
+                         const qfx_index_map* a_rows, const qfx_index_map* a_cols,
+                         const qfx_c64* B, const qfx_index_map* b_rows,
+                         const qfx_index_map* b_cols, qfx_c64* C);
+
 /* trans == 0: y[m] = A[m x k] . x[k]  (cblas_cgemv NoTrans in _multiply_Mv,
  *             src/tensor.cpp:982-1002)
  * trans == 1: y[k] = sum_r x[r] A[r][.], A is [m x k], x has m entries

@@ -19,6 +19,31 @@ KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy", "cgem
 _c = ctypes
 _vp = _c.c_void_p
 _u64 = _c.c_uint64
+
+
+class IndexMap(ctypes.Structure):
+    """qfx_index_map: digits of a matrix index, least significant first."""
+    _fields_ = [("ndigits", ctypes.c_int), ("bits", ctypes.c_ubyte * 16),
+                ("shift", ctypes.c_ubyte * 16)]
+
+
+def index_map(dims, axes):
+    """Map of the matrix index over tensor axes `axes` (most significant
+    first) of a row-major tensor with power-of-two `dims`."""
+    lg = [int(d).bit_length() - 1 for d in dims]
+    if any(1 << b != int(d) for b, d in zip(lg, dims)):
+        raise ValueError("index_map needs power-of-two dimensions")
+    strides = [sum(lg[a + 1:]) for a in range(len(dims))]
+    m = IndexMap()
+    digits = [a for a in reversed(list(axes)) if lg[a] > 0]
+    if len(digits) > 16:
+        raise ValueError("at most 16 digits")
+    m.ndigits = len(digits)
+    for j, a in enumerate(digits):
+        m.bits[j] = lg[a]
+        m.shift[j] = strides[a]
+    return m
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "qfx_last_error": (_c.c_char_p, []),
@@ -35,6 +60,9 @@ SIGNATURES = {
     "qfx_reorder_c64": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.POINTER(_u64),
                                    _c.POINTER(_c.c_int)]),
     "qfx_cgemm_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _u64, _vp, _vp, _vp]),
+    "qfx_cgemm_mapped_c64": (_c.c_int, [_vp, _c.c_int, _vp, _c.POINTER(IndexMap),
+                                        _c.POINTER(IndexMap), _vp, _c.POINTER(IndexMap),
+                                        _c.POINTER(IndexMap), _vp]),
     "qfx_cgemv_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _vp, _vp, _vp]),
     "qfx_cdotu_c64": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
     "qfx_accumulate_c128": (_c.c_int, [_vp, _vp, _u64, _vp, _u64]),
@@ -128,6 +156,16 @@ class Context:
 
     def cgemm(self, precision, m, n, k, A, B, C):
         check(load().qfx_cgemm_c64(self.handle, precision, m, n, k, A.ptr, B.ptr, C.ptr))
+
+    def cgemm_mapped(self, precision, A, a_rows, a_cols, B, b_rows, b_cols, C):
+        """Returns False (C untouched) when no kernel reads this form."""
+        st = load().qfx_cgemm_mapped_c64(self.handle, precision, A.ptr, ctypes.byref(a_rows),
+                                         ctypes.byref(a_cols), B.ptr, ctypes.byref(b_rows),
+                                         ctypes.byref(b_cols), C.ptr)
+        if st == 4:  # QFX_ERR_UNSUPPORTED
+            return False
+        check(st)
+        return True
 
     def cgemv(self, trans, m, k, A, x, y):
         check(load().qfx_cgemv_c64(self.handle, trans, m, k, A.ptr, x.ptr, y.ptr))

@@ -335,6 +335,68 @@ ContractionData ContractionData::Initialize(const std::list<ContractionOperation
 
 // ------------------------------------------------------------ execution
 
+// Digit map (see qfx_index_map) of the matrix index over the named axes
+// `axes` (most significant first) of a row-major tensor; false when a
+// dimension is not a power of two or there are too many digits.
+static bool build_index_map(const std::vector<std::string>& indices,
+                            const std::vector<std::size_t>& dims,
+                            const std::vector<std::string>& axes, qfx_index_map* map) {
+  const std::size_t r = indices.size();
+  std::vector<int> lg(r), shift(r);
+  for (std::size_t i = 0; i < r; ++i) {
+    if (dims[i] == 0 || (dims[i] & (dims[i] - 1)) != 0) return false;
+    int b = 0;
+    while ((std::size_t(1) << b) < dims[i]) ++b;
+    lg[i] = b;
+  }
+  int acc = 0;
+  for (std::size_t i = r; i-- > 0;) {
+    shift[i] = acc;
+    acc += lg[i];
+  }
+  *map = qfx_index_map{};
+  for (auto it = axes.rbegin(); it != axes.rend(); ++it) {
+    const std::size_t p = std::size_t(std::find(indices.begin(), indices.end(), *it) - indices.begin());
+    if (p == r) return false;
+    if (lg[p] == 0) continue;
+    if (map->ndigits == 16) return false;
+    map->bits[map->ndigits] = static_cast<unsigned char>(lg[p]);
+    map->shift[map->ndigits] = static_cast<unsigned char>(shift[p]);
+    map->ndigits++;
+  }
+  return true;
+}
+
+bool ContractionData::try_mapped(const s_type* a, const std::vector<std::string>& a_indices,
+                                 const std::vector<std::size_t>& a_dims, const s_type* b,
+                                 const std::vector<std::string>& b_indices,
+                                 const std::vector<std::size_t>& b_dims,
+                                 const std::vector<std::string>& left,
+                                 const std::vector<std::string>& common,
+                                 const std::vector<std::string>& right, s_type* out) {
+  qfx_index_map ar, ac, br, bc;
+  if (!build_index_map(a_indices, a_dims, left, &ar) ||
+      !build_index_map(a_indices, a_dims, common, &ac) ||
+      !build_index_map(b_indices, b_dims, common, &br) ||
+      !build_index_map(b_indices, b_dims, right, &bc))
+    return false;
+  const int st = qfx_cgemm_mapped_c64(device(), global::precision,
+                                      reinterpret_cast<const qfx_c64*>(a), &ar, &ac,
+                                      reinterpret_cast<const qfx_c64*>(b), &br, &bc,
+                                      reinterpret_cast<qfx_c64*>(out));
+  if (st == QFX_ERR_UNSUPPORTED) return false;
+  qfx_check(st, "qfx_cgemm_mapped_c64");
+  if (global::verbose > 1) {
+    std::size_t m = std::size_t(1) << 0, n = 1, k = 1;
+    for (int j = 0; j < ar.ndigits; ++j) m <<= ar.bits[j];
+    for (int j = 0; j < bc.ndigits; ++j) n <<= bc.bits[j];
+    for (int j = 0; j < ac.ndigits; ++j) k <<= ac.bits[j];
+    std::cerr << "multiply m=" << m << " n=" << n << " k=" << k << " mapped" << std::endl;
+  }
+  stats_.mapped_multiplies++;
+  return true;
+}
+
 ContractionData::View ContractionData::contract_pair(const View& A, const View& B, s_type* out) {
   std::vector<std::string> left = _vector_subtraction(A.indices, B.indices);
   std::vector<std::string> right = _vector_subtraction(B.indices, A.indices);
@@ -354,26 +416,48 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
   const s_type* pa = A.ptr;
   const s_type* pb = B.ptr;
   auto a_order = _vector_union(left, common);
-  if (a_order != A.indices) {
-    s_type* buf = get_scratch(kGeneralSpace).data();
-    device_reorder(A.ptr, buf, A.indices, A.dims, a_order);
-    pa = buf;
-    stats_.reorders++;
-    stats_.reorder_elements += double(A.size());
-  }
   auto b_order = _vector_union(common, right);
-  if (b_order != B.indices) {
-    s_type* buf = get_scratch(kGeneralSpaceB).data();
-    device_reorder(B.ptr, buf, B.indices, B.dims, b_order);
-    pb = buf;
-    stats_.reorders++;
-    stats_.reorder_elements += double(B.size());
+  std::vector<std::string> b_indices = B.indices;
+  std::vector<std::size_t> b_dims = B.dims;
+  bool done = false;
+  if ((a_order != A.indices || b_order != B.indices) && !left.empty() && !right.empty() &&
+      global::mapped_operands) {
+    // Read A (and B) in stored order; if B's layout is the obstacle and B is
+    // the smaller operand, reorder only B and try again.
+    done = try_mapped(A.ptr, A.indices, A.dims, pb, b_indices, b_dims, left, common, right, out);
+    if (!done && b_order != B.indices && B.size() <= A.size()) {
+      s_type* buf = get_scratch(kGeneralSpaceB).data();
+      device_reorder(B.ptr, buf, B.indices, B.dims, b_order);
+      pb = buf;
+      b_indices = b_order;
+      b_dims.clear();
+      for (const auto& x : b_order) b_dims.push_back(bdim[x]);
+      stats_.reorders++;
+      stats_.reorder_elements += double(B.size());
+      done = try_mapped(A.ptr, A.indices, A.dims, pb, b_indices, b_dims, left, common, right, out);
+    }
   }
-  if (global::verbose > 1)
-    std::cerr << "multiply m=" << left_dim << " n=" << right_dim << " k=" << common_dim
-              << " reorder_a=" << (pa != A.ptr ? A.size() : 0)
-              << " reorder_b=" << (pb != B.ptr ? B.size() : 0) << std::endl;
-  device_multiply(pa, pb, out, left_dim, right_dim, common_dim, !left.empty(), !right.empty());
+  if (!done) {
+    if (a_order != A.indices) {
+      s_type* buf = get_scratch(kGeneralSpace).data();
+      device_reorder(A.ptr, buf, A.indices, A.dims, a_order);
+      pa = buf;
+      stats_.reorders++;
+      stats_.reorder_elements += double(A.size());
+    }
+    if (b_order != b_indices) {
+      s_type* buf = get_scratch(kGeneralSpaceB).data();
+      device_reorder(B.ptr, buf, B.indices, B.dims, b_order);
+      pb = buf;
+      stats_.reorders++;
+      stats_.reorder_elements += double(B.size());
+    }
+    if (global::verbose > 1)
+      std::cerr << "multiply m=" << left_dim << " n=" << right_dim << " k=" << common_dim
+                << " reorder_a=" << (pa != A.ptr ? A.size() : 0)
+                << " reorder_b=" << (pb != B.ptr ? B.size() : 0) << std::endl;
+    device_multiply(pa, pb, out, left_dim, right_dim, common_dim, !left.empty(), !right.empty());
+  }
   stats_.multiplies++;
   stats_.flops += 8.0 * double(left_dim) * double(right_dim) * double(common_dim);
   View C;

@@ -86,6 +86,7 @@ struct ContractionStats {
   double flops = 0;
   double reorder_elements = 0;
   std::size_t reorders = 0;
+  std::size_t mapped_multiplies = 0;  // contractions read without reorder
   std::size_t planned_bytes = 0;
 };
 
@@ -146,6 +147,11 @@ class ContractionData {
   void run(std::list<ContractionOperation>& ordering, std::size_t output_index,
            std::unordered_map<std::string, bool>& active_patches,
            std::vector<OpenOutput> open_outputs);
+  bool try_mapped(const s_type* a, const std::vector<std::string>& a_indices,
+                  const std::vector<std::size_t>& a_dims, const s_type* b,
+                  const std::vector<std::string>& b_indices, const std::vector<std::size_t>& b_dims,
+                  const std::vector<std::string>& left, const std::vector<std::string>& common,
+                  const std::vector<std::string>& right, s_type* out);
   View contract_pair(const View& A, const View& B, s_type* out);
   View project_view(const View& v, const std::string& index, std::size_t value,
                     const std::vector<std::size_t>* gather_values, s_type* gather_buffer);

@@ -17,6 +17,11 @@ inline std::size_t memory_limit = std::size_t(1) << 30;
 // qfx_precision values of include/qfx.h (0 = FP32 MFMA).
 inline int precision = 0;
 
+// When set, contractions whose operands are not yet in (left, common) /
+// (common, right) order are read in their stored order by the mapped cGEMM
+// (no reorder) wherever a kernel supports the form.
+inline bool mapped_operands = true;
+
 }  // namespace qflex::global
 
 #endif

@@ -406,9 +406,11 @@ PYBIND11_MODULE(qflex, m) {
     d["flops"] = s.flops;
     d["reorders"] = s.reorders;
     d["reorder_elements"] = s.reorder_elements;
+    d["mapped_multiplies"] = s.mapped_multiplies;
     d["planned_bytes"] = s.planned_bytes;
     return d;
   });
+  m.def("set_mapped_operands", [](bool on) { qflex::global::mapped_operands = on; });
   m.def("set_precision", [](std::string p) {
     wrap([&] {
       qflex::global::precision = parse_precision(p);

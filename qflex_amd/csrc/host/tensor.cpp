@@ -289,6 +289,13 @@ void device_reorder(const s_type* src, s_type* dst, const std::vector<std::strin
   for (int k = 0; k < r; ++k)
     perm[k] = int(std::find(old_ordering.begin(), old_ordering.end(), new_ordering[k]) -
                   old_ordering.begin());
+  if (global::verbose > 1) {
+    std::cerr << "reorder dims=";
+    for (int k = 0; k < r; ++k) std::cerr << (k ? "," : "") << dims[k];
+    std::cerr << " perm=";
+    for (int k = 0; k < r; ++k) std::cerr << (k ? "," : "") << perm[k];
+    std::cerr << std::endl;
+  }
   qfx_check(qfx_reorder_c64(device(), c64(src), c64(dst), r, dims.data(), perm.data()),
             "qfx_reorder_c64");
 }

@@ -553,22 +553,42 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-template <int MODE, int WM, int WN>
-__global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __restrict__ A,
-                                                              const float2* __restrict__ B,
-                                                              float2* __restrict__ C, int M,
-                                                              int N, int K,
-                                                              const unsigned int* __restrict__ amax) {
+// Offset of matrix index idx under a digit map (see qfx_index_map).  The
+// map is a bit permutation, so for i0 a multiple of 2^b and d < 2^b,
+// offset(i0 + d) = offset(i0) + offset(d).
+__device__ __forceinline__ uint64_t map_offset(const qfx_index_map& d, uint64_t idx) {
+  uint64_t off = 0;
+  for (int j = 0; j < d.ndigits; ++j) {
+    const int b = d.bits[j];
+    off += (idx & ((uint64_t(1) << b) - 1)) << d.shift[j];
+    idx >>= b;
+  }
+  return off;
+}
+
+// Operand addressing of the large-tile kernel.
+//   AM 0: A row-major m x k, B row-major k x n.
+//   AM 1: A and B through digit maps; A read along k (a_cols digit 0 at
+//         stride 1), B along n.
+//   AM 2: as 1, but A read along m (a_rows digit 0 at stride 1): each lane
+//         loads rows m, m+1 at one k and stores them as two LDS rows.
+// J = MFMA column tiles per wave (4: 64 complex columns, 2: 32).
+template <int MODE, int WM, int WN, int J, int AM>
+__global__ __launch_bounds__(512) void cgemm_split_big_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
+    int N, int K, const unsigned int* __restrict__ amax, qfx_index_map a_rows,
+    qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols) {
   static_assert(WM * WN == 8, "8 waves");
   typedef typename SplitTraits<MODE>::T T;
   typedef typename SplitTraits<MODE>::V V;
   constexpr int NT = SplitTraits<MODE>::NT;
-  constexpr int kBM = 64 * WM, kBN = 64 * WN;
+  constexpr int kBM = 64 * WM, kBN = 16 * J * WN;
   constexpr int kAPlane = kBM * kRowL;
   constexpr int kBPlane = 2 * kBN * kRowL;
   constexpr int kBuf = NT * (kAPlane + kBPlane);
-  constexpr int kAH = kBM * 4 / 512;  // float4 (2 complex) A loads per thread and stage
-  constexpr int kBH = kBN * 4 / 512;  // float4 B loads per thread and stage
+  constexpr int kAH = kBM * 4 / 512;            // float4 A loads per thread and stage
+  constexpr int kBT = 4 * kBN;                  // float4 B loads per stage
+  constexpr int kBH = (kBT + 511) / 512;        // ... per thread
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
 
@@ -576,7 +596,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave / WN;  // 64 rows each
-  const int wn = wave % WN;  // 64 complex columns each
+  const int wn = wave % WN;  // 16*J complex columns each
   const int m0 = blockIdx.x * kBM;
   const int n0 = blockIdx.y * kBN;
   float sa = 1.f, sb = 1.f;
@@ -585,29 +605,72 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
     sb = fp16_scale(amax[1]);
   }
 
-  // global -> register assignment
-  // A: kBM rows x 8 k = 4*kBM float4 (2 complex each); f = tid + 512*h:
-  //    row = f >> 2, q = f & 3 (complex k = 2q, 2q+1)
-  // B: 8 k x kBN n = 4*kBN float4; g = tid + 512*h: k = g & 7 (consecutive
-  //    lanes take consecutive k, so the LDS stores are conflict-free),
-  //    nq = g >> 3 (complex n = 2nq, 2nq+1)
-  auto load_tile = [&](int kt, float4* ra, float4* rb) {
-    const int k0 = kt * kBBK;
+  // global -> register assignment (f = tid + 512*h, g = tid + 512*h)
+  // A, AM 0/1: row = f >> 2, q = f & 3 (complex k = 2q, 2q+1 of the stage)
+  // A, AM 2:   kk = f & 7, row = 2*(f >> 3) (complex rows row, row+1)
+  // B: kk = g & 7 (consecutive lanes take consecutive k, so the LDS stores
+  //    are conflict-free), nq = g >> 3 (complex n = 2nq, 2nq+1)
+  uint64_t a_off[kAH], a_coff = 0, b_off[kBH], b_roff = 0;
+  if constexpr (AM != 0) {
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + 512 * h;
-      const int row = f >> 2, q = f & 3;
-      const int gm = m0 + row, gk = k0 + 2 * q;
-      ra[h] = (gm < M && gk < K) ? *reinterpret_cast<const float4*>(A + uint64_t(gm) * K + gk)
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int row = AM == 1 ? (f >> 2) : 2 * (f >> 3);
+      a_off[h] = map_offset(a_rows, uint64_t(m0 + row));
+    }
+    a_coff = map_offset(a_cols, AM == 1 ? uint64_t(2 * (tid & 3)) : uint64_t(tid & 7));
+#pragma unroll
+    for (int h = 0; h < kBH; ++h) {
+      const int g = tid + 512 * h;
+      b_off[h] = map_offset(b_cols, uint64_t(n0 + 2 * (g >> 3)));
+    }
+    b_roff = map_offset(b_rows, uint64_t(tid & 7));
+  }
+  // Per-stage k offsets of A and B (uniform), behind the two LDS buffers,
+  // so that the pipelined stage has no digit arithmetic.
+  uint64_t* ktab = reinterpret_cast<uint64_t*>(smem_raw + 2 * kBuf * sizeof(T));
+  const int nkt = (K + kBBK - 1) / kBBK;
+  if constexpr (AM != 0) {
+    for (int t = tid; t < nkt; t += 512) {
+      ktab[2 * t] = map_offset(a_cols, uint64_t(t) * kBBK);
+      ktab[2 * t + 1] = map_offset(b_rows, uint64_t(t) * kBBK);
+    }
+    __syncthreads();
+  }
+  auto load_tile = [&](int kt, float4* ra, float4* rb) {
+    const int k0 = kt * kBBK;
+    uint64_t a_kbase = 0, b_kbase = 0;
+    if constexpr (AM != 0) {
+      a_kbase = ktab[2 * kt];
+      b_kbase = ktab[2 * kt + 1];
+    }
+#pragma unroll
+    for (int h = 0; h < kAH; ++h) {
+      const int f = tid + 512 * h;
+      if constexpr (AM == 2) {
+        const int row = 2 * (f >> 3), kk = f & 7;
+        const int gm = m0 + row, gk = k0 + kk;
+        ra[h] = (gm < M && gk < K)
+                    ? *reinterpret_cast<const float4*>(A + a_off[h] + a_kbase + a_coff)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+      } else {
+        const int row = f >> 2, q = f & 3;
+        const int gm = m0 + row, gk = k0 + 2 * q;
+        const float2* src = AM == 0 ? A + uint64_t(gm) * K + gk : A + a_off[h] + a_kbase + a_coff;
+        ra[h] = (gm < M && gk < K) ? *reinterpret_cast<const float4*>(src)
+                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + 512 * h;
       const int kk = g & 7, nq = g >> 3;
       const int gk = k0 + kk, gn = n0 + 2 * nq;
-      rb[h] = (gk < K && gn < N) ? *reinterpret_cast<const float4*>(B + uint64_t(gk) * N + gn)
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float2* src =
+          AM == 0 ? B + uint64_t(gk) * N + gn : B + b_kbase + b_roff + b_off[h];
+      rb[h] = ((kBT % 512 == 0 || g < kBT) && gk < K && gn < N)
+                  ? *reinterpret_cast<const float4*>(src)
+                  : make_float4(0.f, 0.f, 0.f, 0.f);
     }
   };
   typedef T T2 __attribute__((ext_vector_type(2)));
@@ -617,21 +680,34 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + 512 * h;
-      const int row = f >> 2, q = f & 3;
       float v[4] = {ra[h].x * sa, ra[h].y * sa, ra[h].z * sa, ra[h].w * sa};
       T t3[4][3];
 #pragma unroll
       for (int e = 0; e < 4; ++e) split_terms<MODE>(v[e], t3[e]);
+      if constexpr (AM == 2) {
+        // (re, im) of rows row and row+1 at complex k = kk
+        const int row = 2 * (f >> 3), kk = f & 7;
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        T4 w = {t3[0][t], t3[1][t], t3[2][t], t3[3][t]};
-        *reinterpret_cast<T4*>(base + t * kAPlane + row * kRowL + 4 * q) = w;
+        for (int t = 0; t < NT; ++t) {
+          T2 w0 = {t3[0][t], t3[1][t]};
+          T2 w1 = {t3[2][t], t3[3][t]};
+          *reinterpret_cast<T2*>(base + t * kAPlane + row * kRowL + 2 * kk) = w0;
+          *reinterpret_cast<T2*>(base + t * kAPlane + (row + 1) * kRowL + 2 * kk) = w1;
+        }
+      } else {
+        const int row = f >> 2, q = f & 3;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          T4 w = {t3[0][t], t3[1][t], t3[2][t], t3[3][t]};
+          *reinterpret_cast<T4*>(base + t * kAPlane + row * kRowL + 4 * q) = w;
+        }
       }
     }
     T* bb = base + NT * kAPlane;
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + 512 * h;
+      if (kBT % 512 != 0 && g >= kBT) continue;
       const int kk = g & 7, nq = g >> 3;
       // (Br, Bi) of n, then of n+1
       float v[4] = {rb[h].x * sb, rb[h].y * sb, rb[h].z * sb, rb[h].w * sb};
@@ -657,18 +733,18 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
   const int col = lane & 31;
   const int part = col & 1;
 
-  floatx16 acc[2][4];
+  floatx16 acc[2][J];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < J; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
   auto compute = [&](int buf) {
     const T* base = smem + buf * kBuf;
     const T* bb = base + NT * kAPlane;
-    V a[NT][2], b[NT][4];
+    V a[NT][2], b[NT][J];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
 #pragma unroll
@@ -676,8 +752,8 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
         a[t][i] = *reinterpret_cast<const V*>(base + t * kAPlane +
                                               (wm * 64 + i * 32 + col) * kRowL + 8 * half);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int rc = wn * 128 + j * 32 + col;  // real column of Bhat in the block
+      for (int j = 0; j < J; ++j) {
+        const int rc = wn * 32 * J + j * 32 + col;  // real column of Bhat in the block
         const int row = (rc & 1) * kBN + (rc >> 1);
         b[t][j] = *reinterpret_cast<const V*>(bb + t * kBPlane + row * kRowL + 8 * half);
       }
@@ -685,7 +761,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
+      for (int j = 0; j < J; ++j) {
         if constexpr (SplitTraits<MODE>::PRODUCTS == 6) {
           acc[i][j] = mfma16<MODE>(a[2][i], b[0][j], acc[i][j]);
           acc[i][j] = mfma16<MODE>(a[0][i], b[2][j], acc[i][j]);
@@ -699,9 +775,9 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
   // Interleave of one stage: global loads, fragment reads, then MFMAs in
   // pairs around groups of VALU (operand split) and one LDS store.
   auto schedule = [&]() {
-    constexpr int kMfma = 8 * SplitTraits<MODE>::PRODUCTS;
+    constexpr int kMfma = 2 * J * SplitTraits<MODE>::PRODUCTS;
     __builtin_amdgcn_sched_group_barrier(0x020, kAH + kBH, 0);  // VMEM read
-    __builtin_amdgcn_sched_group_barrier(0x100, 6 * NT, 0);     // DS read
+    __builtin_amdgcn_sched_group_barrier(0x100, (2 + J) * NT, 0);  // DS read
 #pragma unroll
     for (int i = 0; i < kMfma / 2; ++i) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
@@ -711,7 +787,6 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
     }
   };
 
-  const int nkt = (K + kBBK - 1) / kBBK;
   float4 ra0[kAH], rb0[kBH], ra1[kAH], rb1[kBH];
   load_tile(0, ra0, rb0);
   store_tile(0, ra0, rb0);
@@ -739,38 +814,104 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(const float2* __re
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < J; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
-        const int cn = n0 + wn * 64 + j * 16 + (col >> 1);
+        const int cn = n0 + wn * 16 * J + j * 16 + (col >> 1);
         if (row < M && cn < N) Cf[(uint64_t(row) * N + cn) * 2 + part] = acc[i][j][r] * unscale;
       }
 }
 
-template <int MODE, int WM, int WN>
-size_t split_big_smem() {
-  return size_t(2) * SplitTraits<MODE>::NT * (64 * WM * kRowL + 2 * 64 * WN * kRowL) * 2;
+template <int MODE, int WM, int WN, int J>
+constexpr size_t split_big_smem() {
+  return size_t(2) * SplitTraits<MODE>::NT * (64 * WM * kRowL + 2 * 16 * J * WN * kRowL) * 2;
+}
+
+constexpr size_t kMaxLds = 160 * 1024;
+
+// LDS of one launch: the two stage buffers, plus 16 bytes per stage for the
+// per-stage k offsets of the mapped forms.
+template <int MODE, int WM, int WN, int J, int AM>
+size_t split_big_lds(uint64_t k) {
+  return split_big_smem<MODE, WM, WN, J>() + (AM != 0 ? 16 * ((k + kBBK - 1) / kBBK) : 0);
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel once per process.
-template <int MODE, int WM, int WN>
+template <int MODE, int WM, int WN, int J, int AM>
 hipError_t split_big_prepare() {
   static hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, int(split_big_smem<MODE, WM, WN>()));
+      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN, J, AM>),
+      hipFuncAttributeMaxDynamicSharedMemorySize,
+      int(AM != 0 ? kMaxLds : split_big_smem<MODE, WM, WN, J>()));
   return e;
 }
 
-template <int MODE, int WM, int WN>
+template <int MODE, int WM, int WN, int J, int AM>
 hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
-                            uint64_t m, uint64_t n, uint64_t k, const unsigned int* amax) {
-  hipError_t e = split_big_prepare<MODE, WM, WN>();
+                            uint64_t m, uint64_t n, uint64_t k, const unsigned int* amax,
+                            const qfx_index_map* maps) {
+  hipError_t e = split_big_prepare<MODE, WM, WN, J, AM>();
   if (e != hipSuccess) return e;
-  dim3 g{unsigned((m + 64 * WM - 1) / (64 * WM)), unsigned((n + 64 * WN - 1) / (64 * WN)), 1};
-  cgemm_split_big_kernel<MODE, WM, WN><<<g, 512, split_big_smem<MODE, WM, WN>(), stream>>>(
-      A, B, C, int(m), int(n), int(k), amax);
+  constexpr int kBM = 64 * WM, kBN = 16 * J * WN;
+  dim3 g{unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN), 1};
+  qfx_index_map none{};
+  cgemm_split_big_kernel<MODE, WM, WN, J, AM>
+      <<<g, 512, split_big_lds<MODE, WM, WN, J, AM>(k), stream>>>(
+          A, B, C, int(m), int(n), int(k), amax, maps ? maps[0] : none, maps ? maps[1] : none,
+          maps ? maps[2] : none, maps ? maps[3] : none);
   return hipGetLastError();
+}
+
+// Tile layouts of the large-tile kernel (8 waves as WM x WN, J column tiles
+// per wave): 4x2 (256 x 128 complex), 2x4 (128 x 256), 8x1 with J = 2
+// (512 x 32, for n < 128).
+enum BigLayout { kLayoutNone = 0, kLayout4x2, kLayout2x4, kLayout8x1 };
+
+BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
+  auto blocks = [&](uint64_t bm, uint64_t bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
+  if (n >= 512 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(128, 256) >= 256) return kLayout2x4;
+  if (n >= 128 && blocks(256, 128) >= 256) return kLayout4x2;
+  // three BF16 planes of the 512-row tile would exceed the 160 KiB of LDS
+  if (n >= 32 && n < 128 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(512, 32) >= 256)
+    return kLayout8x1;
+  return kLayoutNone;
+}
+
+template <int MODE, int AM>
+hipError_t launch_split_big_layout(BigLayout lay, hipStream_t stream, const float2* A,
+                                   const float2* B, float2* C, uint64_t m, uint64_t n,
+                                   uint64_t k, const unsigned int* amax,
+                                   const qfx_index_map* maps) {
+  if constexpr (MODE == 0) {
+    return launch_split_big<0, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
+  } else {
+    if (lay == kLayout2x4)
+      return launch_split_big<MODE, 2, 4, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
+    if (lay == kLayout8x1)
+      return launch_split_big<MODE, 8, 1, 2, AM>(stream, A, B, C, m, n, k, amax, maps);
+    return launch_split_big<MODE, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
+  }
+}
+
+// Whether the mapped form's per-stage table fits beside the stage buffers.
+bool split_big_mapped_fits(int precision, BigLayout lay, uint64_t k) {
+  if (precision == QFX_PREC_SPLIT_BF16X3) return split_big_lds<0, 4, 2, 4, 1>(k) <= kMaxLds;
+  if (lay == kLayout2x4) return split_big_lds<2, 2, 4, 4, 1>(k) <= kMaxLds;
+  if (lay == kLayout8x1) return split_big_lds<2, 8, 1, 2, 1>(k) <= kMaxLds;
+  return split_big_lds<2, 4, 2, 4, 1>(k) <= kMaxLds;
+}
+
+template <int AM>
+hipError_t launch_split_big_prec(int precision, BigLayout lay, hipStream_t stream,
+                                 const float2* A, const float2* B, float2* C, uint64_t m,
+                                 uint64_t n, uint64_t k, const unsigned int* amax,
+                                 const qfx_index_map* maps) {
+  if (precision == QFX_PREC_SPLIT_BF16X3)
+    return launch_split_big_layout<0, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
+  if (precision == QFX_PREC_SPLIT_BF16)
+    return launch_split_big_layout<1, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
+  return launch_split_big_layout<2, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
 }
 
 // ------------------------------------------------------------ GEMV / dot
@@ -992,26 +1133,58 @@ __global__ __launch_bounds__(256) void cgemm_tiny_mn_partial_kernel(
 // (BM = 4096 / BN), each thread 4 rows x 4 columns; A and B tiles staged
 // through LDS with coalesced loads.
 constexpr int kSmallK = 16;
-template <int BN>
-__global__ __launch_bounds__(256) void cgemm_smallk_kernel(const float2* __restrict__ A,
-                                                           const float2* __restrict__ B,
-                                                           float2* __restrict__ C, uint64_t M,
-                                                           int N, int K) {
+// MAPPED: operands through digit maps (offset tables in LDS); A is read
+// along m when a_mfast (a_rows digit 0 at stride 1), else along k; B along
+// n when b_nfast, else along k.
+template <int BN, bool MAPPED>
+__global__ __launch_bounds__(256) void cgemm_smallk_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
+    uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
+    qfx_index_map b_cols, bool a_mfast, bool b_nfast) {
   constexpr int TX = BN / 4;        // column groups
   constexpr int BM = 4 * 256 / TX;  // rows per block
   __shared__ float2 As[BM][kSmallK + 1];
   __shared__ float2 Bs[kSmallK][BN];
+  __shared__ uint64_t arow[MAPPED ? BM : 1], bcol[MAPPED ? BN : 1];
+  __shared__ uint64_t acol[MAPPED ? kSmallK : 1], brow[MAPPED ? kSmallK : 1];
   const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
   const uint64_t m0 = uint64_t(blockIdx.x) * BM;
   const int n0 = blockIdx.y * BN;
-  // A tile: rows m0.., all K columns (contiguous in memory)
+  if constexpr (MAPPED) {
+    for (int i = threadIdx.x; i < BM; i += 256) arow[i] = map_offset(a_rows, m0 + i);
+    for (int i = threadIdx.x; i < BN; i += 256) bcol[i] = map_offset(b_cols, uint64_t(n0 + i));
+    if (threadIdx.x < K) {
+      acol[threadIdx.x] = map_offset(a_cols, threadIdx.x);
+      brow[threadIdx.x] = map_offset(b_rows, threadIdx.x);
+    }
+    __syncthreads();
+  }
+  // A tile: rows m0.., all K columns
   for (int e = threadIdx.x; e < BM * K; e += 256) {
-    const int r = e / K, c = e - r * K;
-    As[r][c] = m0 + r < M ? A[(m0 + r) * K + c] : make_float2(0.f, 0.f);
+    int r, c;
+    if (MAPPED && a_mfast) {
+      c = e / BM;
+      r = e - c * BM;
+    } else {
+      r = e / K;
+      c = e - r * K;
+    }
+    float2 v = make_float2(0.f, 0.f);
+    if (m0 + r < M) v = MAPPED ? A[arow[r] + acol[c]] : A[(m0 + r) * K + c];
+    As[r][c] = v;
   }
   for (int e = threadIdx.x; e < K * BN; e += 256) {
-    const int r = e / BN, c = e - r * BN;
-    Bs[r][c] = n0 + c < N ? B[uint64_t(r) * N + n0 + c] : make_float2(0.f, 0.f);
+    int r, c;
+    if (MAPPED && !b_nfast) {
+      c = e / K;
+      r = e - c * K;
+    } else {
+      r = e / BN;
+      c = e - r * BN;
+    }
+    float2 v = make_float2(0.f, 0.f);
+    if (n0 + c < N) v = MAPPED ? B[brow[r] + bcol[c]] : B[uint64_t(r) * N + n0 + c];
+    Bs[r][c] = v;
   }
   __syncthreads();
   float2 acc[4][4];
@@ -1039,6 +1212,62 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(const float2* __restr
       const int c = n0 + tx + TX * j;
       if (c < N) C[r * N + c] = acc[i][j];
     }
+  }
+}
+
+// Mapped form of cgemm_tiny_mn_partial_kernel: A(i, k) = A[a_rows(i) +
+// a_cols(k)], B(k, j) = B[b_rows(k) + b_cols(j)]; same order of operations.
+__global__ __launch_bounds__(256) void cgemm_tiny_mn_mapped_partial_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ part,
+    int M, int N, uint64_t K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
+    qfx_index_map b_cols) {
+  __shared__ float2 red[4][64];
+  uint64_t ar[8], bc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) ar[i] = i < M ? map_offset(a_rows, i) : 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) bc[j] = j < N ? map_offset(b_cols, j) : 0;
+  float2 acc[64];
+#pragma unroll
+  for (int o = 0; o < 64; ++o) acc[o] = make_float2(0.f, 0.f);
+  const uint64_t k0 = uint64_t(blockIdx.x) * kTinyChunk;
+  const uint64_t k1 = k0 + kTinyChunk < K ? k0 + kTinyChunk : K;
+  for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
+    const uint64_t ak = map_offset(a_cols, k), bk = map_offset(b_rows, k);
+    float2 a[8], b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = i < M ? A[ar[i] + ak] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) b[j] = j < N ? B[bk + bc[j]] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i * 8 + j] = cfma(a[i], b[j], acc[i * 8 + j]);
+  }
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 32, half = 32; s > 0; s >>= 1, half >>= 1) {
+    const bool upper = (lane & s) != 0;
+#pragma unroll
+    for (int o = 0; o < half; ++o) {
+      float2 send = upper ? acc[o] : acc[o + half];
+      float2 keep = upper ? acc[o + half] : acc[o];
+      keep.x += __shfl_xor(send.x, s);
+      keep.y += __shfl_xor(send.y, s);
+      acc[o] = keep;
+    }
+  }
+  red[threadIdx.x >> 6][lane] = acc[0];
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    float2 t = red[0][lane];
+#pragma unroll
+    for (int w = 1; w < 4; ++w) {
+      t.x += red[w][lane].x;
+      t.y += red[w][lane].y;
+    }
+    const int i = lane >> 3, j = lane & 7;
+    if (i < M && j < N) part[uint64_t(blockIdx.x) * M * N + i * N + j] = t;
   }
 }
 
@@ -1101,6 +1330,22 @@ extern "C" int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, con
   return QFX_OK;
 }
 
+// Operand scales of the split-FP16 form: max|A| and max|B| into a 2-word
+// workspace on the device (no host round trip).
+static int launch_amax(qfx_context* ctx, const float2* A, uint64_t na, const float2* B,
+                       uint64_t nb, unsigned int** amax) {
+  void* ws = nullptr;
+  int st = qfx_workspace(ctx, 16, &ws);
+  if (st != QFX_OK) return st;
+  *amax = static_cast<unsigned int*>(ws);
+  zero_u32_kernel<<<1, 64, 0, ctx->stream>>>(*amax, 2);
+  amax_kernel<<<cap_grid(std::min<uint64_t>((na + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
+      A, na, *amax);
+  amax_kernel<<<cap_grid(std::min<uint64_t>((nb + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
+      B, nb, *amax + 1);
+  return QFX_OK;
+}
+
 extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n,
                              uint64_t k, const qfx_c64* A, const qfx_c64* B, qfx_c64* C) {
   QFX_CHECK_CTX(ctx);
@@ -1122,10 +1367,12 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
-      cgemm_smallk_kernel<16><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k));
+      cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
+                                                                 {}, {}, {}, {}, false, true);
     } else {
       dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
-      cgemm_smallk_kernel<64><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k));
+      cgemm_smallk_kernel<64, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
+                                                                 {}, {}, {}, {}, false, true);
     }
     QFX_HIP(hipGetLastError());
     return QFX_OK;
@@ -1183,40 +1430,22 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
   dim3 grid(unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN));
   if (grid.y > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
-  // Large tiles: 128 x 256 (n >= 512) or 256 x 128 complex, one block per
-  // CU, when there are enough of them to fill the chip.  (Three BF16 planes
-  // of the wide tile would exceed the 160 KiB of LDS.)
-  const bool wide = n >= 512 && precision != QFX_PREC_SPLIT_BF16X3;
-  const uint64_t big_blocks = wide ? ((m + 127) / 128) * ((n + 255) / 256)
-                                   : ((m + 255) / 256) * ((n + 127) / 128);
-  const bool big_tile = n >= 128 && big_blocks >= 256 && (k % 2 == 0) && (n % 2 == 0) &&
-                        (reinterpret_cast<uintptr_t>(A) % 16 == 0) &&
-                        (reinterpret_cast<uintptr_t>(B) % 16 == 0);
+  // Large tiles (one block per CU) when there are enough of them to fill
+  // the chip; float4 operand loads need even k and n and 16-byte alignment.
+  const BigLayout layout =
+      (precision != QFX_PREC_FP32 && (k % 2 == 0) && (n % 2 == 0) &&
+       (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0))
+          ? choose_big_layout(precision, m, n)
+          : kLayoutNone;
   unsigned int* amax = nullptr;
   if (precision == QFX_PREC_SPLIT_FP16) {
-    // operand scales: max|A| and max|B| on the device (no host round trip)
-    void* ws = nullptr;
-    int st = qfx_workspace(ctx, 16, &ws);
+    int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax);
     if (st != QFX_OK) return st;
-    amax = static_cast<unsigned int*>(ws);
-    zero_u32_kernel<<<1, 64, 0, ctx->stream>>>(amax, 2);
-    amax_kernel<<<cap_grid(std::min<uint64_t>((m * k + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
-        A2, m * k, amax);
-    amax_kernel<<<cap_grid(std::min<uint64_t>((k * n + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
-        B2, k * n, amax + 1);
   }
-  if (precision != QFX_PREC_FP32 && big_tile) {
-    if ((n + 127) / 128 > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
-    hipError_t e;
-    if (precision == QFX_PREC_SPLIT_BF16X3)
-      e = launch_split_big<0, 4, 2>(ctx->stream, A2, B2, C2, m, n, k, amax);
-    else if (precision == QFX_PREC_SPLIT_BF16)
-      e = wide ? launch_split_big<1, 2, 4>(ctx->stream, A2, B2, C2, m, n, k, amax)
-               : launch_split_big<1, 4, 2>(ctx->stream, A2, B2, C2, m, n, k, amax);
-    else
-      e = wide ? launch_split_big<2, 2, 4>(ctx->stream, A2, B2, C2, m, n, k, amax)
-               : launch_split_big<2, 4, 2>(ctx->stream, A2, B2, C2, m, n, k, amax);
-    QFX_HIP(e);
+  if (layout != kLayoutNone) {
+    if ((n + 31) / 32 > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
+    QFX_HIP(launch_split_big_prec<0>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
+                                     nullptr));
     return QFX_OK;
   }
   if (precision == QFX_PREC_SPLIT_FP16) {
@@ -1232,5 +1461,101 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   else
     cgemm_bf16_mfma_kernel<6><<<grid, 256, 0, ctx->stream>>>(A2, B2, C2, int(m), int(n), int(k));
   QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
+// ------------------------------------------------------------ mapped operands
+
+static bool map_size(const qfx_index_map* d, uint64_t* size) {
+  if (!d || d->ndigits < 0 || d->ndigits > 16) return false;
+  int total = 0;
+  for (int j = 0; j < d->ndigits; ++j) {
+    if (d->bits[j] == 0 || d->shift[j] > 47) return false;
+    total += d->bits[j];
+  }
+  if (total > 47) return false;
+  *size = uint64_t(1) << total;
+  return true;
+}
+
+// Digit 0 at stride 1 with at least `bits` bits: runs of 2^bits contiguous
+// elements along this matrix dimension.
+static bool contiguous(const qfx_index_map& d, int bits) {
+  return d.ndigits > 0 && d.shift[0] == 0 && d.bits[0] >= bits;
+}
+
+extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c64* A,
+                                    const qfx_index_map* a_rows, const qfx_index_map* a_cols,
+                                    const qfx_c64* B, const qfx_index_map* b_rows,
+                                    const qfx_index_map* b_cols, qfx_c64* C) {
+  QFX_CHECK_CTX(ctx);
+  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_SPLIT_FP16)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: unknown precision");
+  uint64_t m, k, k2, n;
+  if (!map_size(a_rows, &m) || !map_size(a_cols, &k) || !map_size(b_rows, &k2) ||
+      !map_size(b_cols, &n))
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: invalid index map");
+  if (k != k2)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: a_cols and b_rows differ in size");
+  if (!A || !B || !C) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: null pointer");
+  const float2* A2 = reinterpret_cast<const float2*>(A);
+  const float2* B2 = reinterpret_cast<const float2*>(B);
+  float2* C2 = reinterpret_cast<float2*>(C);
+  const double flops = 8.0 * double(m) * double(n) * double(k);
+  const double bytes = 8.0 * (double(m) * k + double(k) * n + double(m) * n);
+
+  if (k <= uint64_t(kSmallK) && n <= 65535ULL * 64) {
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
+    const bool a_mfast = contiguous(*a_rows, 1) && !contiguous(*a_cols, 3);
+    const bool b_nfast = contiguous(*b_cols, 1) || !contiguous(*b_rows, 1);
+    if (n <= 16) {
+      dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
+      cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(
+          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast);
+    } else {
+      dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
+      cgemm_smallk_kernel<64, true><<<g, 256, 0, ctx->stream>>>(
+          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast);
+    }
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
+  if (precision != QFX_PREC_FP32_EXACT && m <= 8 && n <= 8 && k >= 1024) {
+    QfxLaunchScope scope(ctx, QFX_K_GEMV, bytes, flops);
+    const uint64_t nchunk = (k + kTinyChunk - 1) / kTinyChunk;
+    void* ws = nullptr;
+    int st = qfx_workspace(ctx, nchunk * m * n * sizeof(float2), &ws);
+    if (st != QFX_OK) return st;
+    float2* part = static_cast<float2*>(ws);
+    cgemm_tiny_mn_mapped_partial_kernel<<<unsigned(nchunk), 256, 0, ctx->stream>>>(
+        A2, B2, part, int(m), int(n), k, *a_rows, *a_cols, *b_rows, *b_cols);
+    sum_partials_kernel<<<1, 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
+    QFX_HIP(hipGetLastError());
+    return QFX_OK;
+  }
+  const bool split = precision == QFX_PREC_SPLIT_FP16 || precision == QFX_PREC_SPLIT_BF16 ||
+                     precision == QFX_PREC_SPLIT_BF16X3;
+  const BigLayout layout =
+      (split && m <= 0x7fffffffULL && n <= 0x7fffffffULL && k >= 8 && k <= 0x7fffffffULL &&
+       reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0)
+          ? choose_big_layout(precision, m, n)
+          : kLayoutNone;
+  const int am = contiguous(*a_cols, 3) ? 1 : (contiguous(*a_rows, 3) ? 2 : 0);
+  if (layout == kLayoutNone || am == 0 || !contiguous(*b_cols, 3) || (n + 31) / 32 > 65535 ||
+      !split_big_mapped_fits(precision, layout, k))
+    return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_mapped_c64: no kernel reads this form");
+  QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
+  unsigned int* amax = nullptr;
+  if (precision == QFX_PREC_SPLIT_FP16) {
+    int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax);
+    if (st != QFX_OK) return st;
+  }
+  const qfx_index_map maps[4] = {*a_rows, *a_cols, *b_rows, *b_cols};
+  if (am == 1)
+    QFX_HIP(launch_split_big_prec<1>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
+                                     maps));
+  else
+    QFX_HIP(launch_split_big_prec<2>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
+                                     maps));
   return QFX_OK;
 }

@@ -35,3 +35,12 @@ def test_python_binding_covers_header():
 def test_error_string_without_gpu_call():
     lib = capi.load()
     assert lib.qfx_last_error() is not None
+
+
+def test_index_map_layout():
+    # dims (4, 2, 8): strides 16, 8, 1 -> log2 4, 3, 0; digits least significant first
+    m = capi.index_map([4, 2, 8], [0, 2])
+    assert m.ndigits == 2
+    assert list(m.bits[:2]) == [3, 2] and list(m.shift[:2]) == [0, 4]
+    # unit axes contribute no digit
+    assert capi.index_map([1, 16], [0, 1]).ndigits == 1

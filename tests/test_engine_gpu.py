@@ -265,6 +265,29 @@ def test_corner_inward_7x7_ordering_vs_oracle(depth):
     assert relerr([a for _, a in r], [a for _, a in o]) < 1e-5
 
 
+@pytest.mark.parametrize("precision", ["split_fp16", "split_bf16", "fp32"])
+def test_mapped_operands_bit_identical_to_reorders(precision):
+    # Reading operands in stored order (no reorder) feeds the same values in
+    # the same k order to the same kernels: amplitudes must match bit for bit.
+    circ = circuits.random_circuit(7, 7, 16, 5)
+    order = circuits.corner_inward_7x7()
+    grid = circuits.grid_text(7, 7)
+    try:
+        qflex.set_mapped_operands(False)
+        plain = run_engine(circ, order, grid, "0" * 49, "", precision)
+        plain_stats = qflex.last_stats()
+    finally:
+        qflex.set_mapped_operands(True)
+    mapped = run_engine(circ, order, grid, "0" * 49, "", precision)
+    stats = qflex.last_stats()
+    assert plain_stats["mapped_multiplies"] == 0
+    if precision != "fp32":
+        assert stats["mapped_multiplies"] > 0
+        assert stats["reorder_elements"] < plain_stats["reorder_elements"]
+    assert [s for s, _ in mapped] == [s for s, _ in plain]
+    assert all(a == b for (_, a), (_, b) in zip(mapped, plain))
+
+
 def test_memory_limit_error():
     circ = circuits.random_circuit(3, 3, 16, 1)
     with pytest.raises(RuntimeError, match=r"Required space \(.*\) exceeds memory limit \(1 kB\)"):

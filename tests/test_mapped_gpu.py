@@ -1,0 +1,137 @@
+"""GPU parity of qfx_cgemm_mapped_c64: contractions read straight from the
+stored index orders of both tensors (no reorder), against numpy complex128
+of the reordered matrices (the reference's reorder-then-cgemm,
+src/tensor.cpp:1100-1128)."""
+import numpy as np
+import pytest
+
+from qflex_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+TOL = {capi.FP32: 2e-6, capi.FP32_EXACT: 2e-6, capi.SPLIT_BF16X3: 5e-6, capi.SPLIT_BF16: 3e-4,
+       capi.SPLIT_FP16: 5e-6}
+
+
+def make_case(left, common, right, a_inner, b_inner, seed):
+    """Random storage orders of A (left + common axes) and B (common + right).
+
+    left/common/right are lists of dimensions; a_inner / b_inner name the
+    group ("left", "common", "right") whose last axis is stored innermost.
+    Returns dims and the axis lists of the matrix views (most significant
+    axis first; the common axes listed in the same order for A and B).
+    """
+    rng = np.random.default_rng(seed)
+    names_l = [("l", i) for i in range(len(left))]
+    names_c = [("c", i) for i in range(len(common))]
+    names_r = [("r", i) for i in range(len(right))]
+    dim = {**{n: d for n, d in zip(names_l, left)}, **{n: d for n, d in zip(names_c, common)},
+           **{n: d for n, d in zip(names_r, right)}}
+    groups = {"left": names_l, "common": names_c, "right": names_r}
+
+    def order(names, inner):
+        last = groups[inner][-1]
+        rest = [n for n in names if n != last]
+        return [rest[i] for i in rng.permutation(len(rest))] + [last]
+
+    a_names = order(names_l + names_c, a_inner)
+    b_names = order(names_c + names_r, b_inner)
+    a_dims = [dim[n] for n in a_names]
+    b_dims = [dim[n] for n in b_names]
+    return (a_dims, [a_names.index(n) for n in names_l], [a_names.index(n) for n in names_c],
+            b_dims, [b_names.index(n) for n in names_c], [b_names.index(n) for n in names_r])
+
+
+def run(ctx, precs, case, integer=False, seed=0):
+    a_dims, a_left, a_common, b_dims, b_common, b_right = case
+    rng = np.random.default_rng(seed)
+    if integer:
+        a = (rng.integers(-8, 8, a_dims) + 1j * rng.integers(-8, 8, a_dims)).astype(np.complex64)
+        b = (rng.integers(-8, 8, b_dims) + 1j * rng.integers(-8, 8, b_dims)).astype(np.complex64)
+    else:
+        a = (rng.standard_normal(a_dims) + 1j * rng.standard_normal(a_dims)).astype(np.complex64)
+        b = (rng.standard_normal(b_dims) + 1j * rng.standard_normal(b_dims)).astype(np.complex64)
+    m = int(np.prod([a_dims[x] for x in a_left]))
+    k = int(np.prod([a_dims[x] for x in a_common]))
+    n = int(np.prod([b_dims[x] for x in b_right]))
+    am = a.transpose(a_left + a_common).reshape(m, k).astype(np.complex128)
+    bm = b.transpose(b_common + b_right).reshape(k, n).astype(np.complex128)
+    ref = am @ bm
+    maps = (capi.index_map(a_dims, a_left), capi.index_map(a_dims, a_common),
+            capi.index_map(b_dims, b_common), capi.index_map(b_dims, b_right))
+    A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+    out = {}
+    for prec in precs:
+        if not ctx.cgemm_mapped(prec, A, maps[0], maps[1], B, maps[2], maps[3], C):
+            out[prec] = None
+            continue
+        got = ctx.download(C, np.complex64, (m, n))
+        if integer:
+            out[prec] = bool(np.array_equal(got, ref.astype(np.complex64)))
+        else:
+            out[prec] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    for buf in (A, B, C):
+        buf.free()
+    return out
+
+
+SPLITS = [capi.SPLIT_FP16, capi.SPLIT_BF16, capi.SPLIT_BF16X3]
+
+# (left, common, right, a_inner, b_inner): contraction patterns of grid
+# patches (bond dimension 16, terminal outputs 2), at test sizes with at
+# least 256 tiles.
+BIG_CASES = [
+    ([16, 16, 16, 16], [16, 2], [8, 16], "left", "right"),          # 256x128, A along m
+    ([16, 16, 16, 16], [2, 16], [16, 8], "common", "right"),        # 256x128, A along k
+    ([16, 16, 16, 4], [2, 16], [16, 2, 16], "common", "right"),     # 128x256, A along k
+    ([16, 16, 4, 16], [16, 2], [2, 16, 16], "left", "right"),       # 128x256, A along m
+    ([16, 16, 2, 16, 2, 16], [16, 2], [2, 16], "left", "right"),    # 512x32
+]
+
+
+@pytest.mark.parametrize("case", range(len(BIG_CASES)))
+def test_mapped_large_tiles(ctx, case):
+    res = run(ctx, SPLITS, make_case(*BIG_CASES[case], seed=case), seed=case)
+    for prec, err in res.items():
+        if err is None:
+            # BF16x3 has no 512x32 tile (its three planes exceed the LDS)
+            assert prec == capi.SPLIT_BF16X3 and case == 4, (case, prec)
+            continue
+        assert err <= TOL[prec], (case, prec, err)
+
+
+def test_mapped_large_tiles_integer_inputs_bit_exact(ctx):
+    for case in (0, 2, 4):
+        res = run(ctx, SPLITS, make_case(*BIG_CASES[case], seed=case), integer=True, seed=case)
+        for prec, ok in res.items():
+            assert ok is not False, (case, prec)
+
+
+SMALL_CASES = [
+    ([16, 16, 16, 16], [16], [16], "left", "right"),         # k = 16, n = 16, A along m
+    ([16, 16, 16], [16], [4, 16], "common", "common"),       # n = 64, A and B along k
+    ([16, 32], [2, 4], [8], "left", "right"),                # k = 8, ragged tiles
+    ([2, 2, 2], [16, 16, 16, 16, 16], [2, 2, 2], "common", "common"),  # 8 x 8 x 2^20
+]
+
+
+@pytest.mark.parametrize("case", range(len(SMALL_CASES)))
+def test_mapped_vector_kernels(ctx, case):
+    precs = [capi.FP32, capi.FP32_EXACT, capi.SPLIT_FP16]
+    res = run(ctx, precs, make_case(*SMALL_CASES[case], seed=20 + case), seed=case)
+    for prec, err in res.items():
+        if case == 3 and prec == capi.FP32_EXACT:
+            assert err is None  # the split-K form has no defined order
+            continue
+        assert err is not None and err <= 2e-6, (case, prec, err)
+
+
+def test_mapped_unsupported_forms(ctx):
+    # FP32 precision at MFMA sizes: the caller reorders instead
+    res = run(ctx, [capi.FP32], make_case(*BIG_CASES[0], seed=0))
+    assert res[capi.FP32] is None
+    # contiguous run of 2 elements along the read axis of A
+    res = run(ctx, [capi.SPLIT_FP16], make_case([16, 16, 16, 16, 2], [16, 2], [8, 16], "left",
+                                                "right", seed=5))
+    assert res[capi.SPLIT_FP16] is None
+
