@@ -870,10 +870,10 @@ enum BigLayout { kLayoutNone = 0, kLayout4x2, kLayout2x4, kLayout8x1 };
 
 BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
   auto blocks = [&](uint64_t bm, uint64_t bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
-  if (n >= 512 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(128, 256) >= 256) return kLayout2x4;
-  if (n >= 128 && blocks(256, 128) >= 256) return kLayout4x2;
+  if (n >= 512 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(128, 256) >= 128) return kLayout2x4;
+  if (n >= 128 && blocks(256, 128) >= 128) return kLayout4x2;
   // three BF16 planes of the 512-row tile would exceed the 160 KiB of LDS
-  if (n >= 32 && n < 128 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(512, 32) >= 256)
+  if (n >= 32 && n < 128 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(512, 32) >= 128)
     return kLayout8x1;
   return kLayoutNone;
 }
@@ -1136,11 +1136,15 @@ constexpr int kSmallK = 16;
 // MAPPED: operands through digit maps (offset tables in LDS); A is read
 // along m when a_mfast (a_rows digit 0 at stride 1), else along k; B along
 // n when b_nfast, else along k.
+// Split-K use (blockIdx.z = chunk > 0 or part != nullptr): the block covers
+// k in [16*z, 16*z + K) of a common dimension of length ldk and writes its
+// partial tile to part[z][m][n] (summed afterwards in fixed order).
 template <int BN, bool MAPPED>
 __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
     uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
-    qfx_index_map b_cols, bool a_mfast, bool b_nfast) {
+    qfx_index_map b_cols, bool a_mfast, bool b_nfast, uint64_t ldk, float2* __restrict__ part) {
+  const uint64_t kbeg = uint64_t(blockIdx.z) * kSmallK;
   constexpr int TX = BN / 4;        // column groups
   constexpr int BM = 4 * 256 / TX;  // rows per block
   __shared__ float2 As[BM][kSmallK + 1];
@@ -1154,8 +1158,8 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
     for (int i = threadIdx.x; i < BM; i += 256) arow[i] = map_offset(a_rows, m0 + i);
     for (int i = threadIdx.x; i < BN; i += 256) bcol[i] = map_offset(b_cols, uint64_t(n0 + i));
     if (threadIdx.x < K) {
-      acol[threadIdx.x] = map_offset(a_cols, threadIdx.x);
-      brow[threadIdx.x] = map_offset(b_rows, threadIdx.x);
+      acol[threadIdx.x] = map_offset(a_cols, kbeg + threadIdx.x);
+      brow[threadIdx.x] = map_offset(b_rows, kbeg + threadIdx.x);
     }
     __syncthreads();
   }
@@ -1170,7 +1174,7 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
       c = e - r * K;
     }
     float2 v = make_float2(0.f, 0.f);
-    if (m0 + r < M) v = MAPPED ? A[arow[r] + acol[c]] : A[(m0 + r) * K + c];
+    if (m0 + r < M) v = MAPPED ? A[arow[r] + acol[c]] : A[(m0 + r) * ldk + kbeg + c];
     As[r][c] = v;
   }
   for (int e = threadIdx.x; e < K * BN; e += 256) {
@@ -1183,7 +1187,7 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
       c = e - r * BN;
     }
     float2 v = make_float2(0.f, 0.f);
-    if (n0 + c < N) v = MAPPED ? B[brow[r] + bcol[c]] : B[uint64_t(r) * N + n0 + c];
+    if (n0 + c < N) v = MAPPED ? B[brow[r] + bcol[c]] : B[(kbeg + r) * N + n0 + c];
     Bs[r][c] = v;
   }
   __syncthreads();
@@ -1203,6 +1207,7 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = cfma(a[i], b[j], acc[i][j]);
   }
+  float2* out = part ? part + uint64_t(blockIdx.z) * M * N : C;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint64_t r = m0 + ty * 4 + i;
@@ -1210,10 +1215,11 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int c = n0 + tx + TX * j;
-      if (c < N) C[r * N + c] = acc[i][j];
+      if (c < N) out[r * N + c] = acc[i][j];
     }
   }
 }
+
 
 // Mapped form of cgemm_tiny_mn_partial_kernel: A(i, k) = A[a_rows(i) +
 // a_cols(k)], B(k, j) = B[b_rows(k) + b_cols(j)]; same order of operations.
@@ -1274,6 +1280,52 @@ __global__ __launch_bounds__(256) void cgemm_tiny_mn_mapped_partial_kernel(
 unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, 65535))); }
 
 }  // namespace
+
+// Small-output GEMMs with a longer common dimension (m*n <= 2^16,
+// 16 < k <= 4096, k a multiple of 16): split-K over chunks of 16 with the
+// vector-ALU tile kernel, partial tiles summed in fixed chunk order.
+// Enough blocks to fill the chip where one MFMA tile per output block
+// would leave most CUs idle.
+static bool smallk_splitk_fits(uint64_t m, uint64_t n, uint64_t k) {
+  return m * n <= (uint64_t(1) << 16) && k > uint64_t(kSmallK) && k <= 4096 &&
+         k % kSmallK == 0;
+}
+
+static int launch_smallk_splitk(qfx_context* ctx, const float2* A, const float2* B, float2* C,
+                                uint64_t m, uint64_t n, uint64_t k, bool mapped,
+                                const qfx_index_map* maps, bool a_mfast, bool b_nfast) {
+  const uint64_t nchunk = k / kSmallK;
+  void* ws = nullptr;
+  int st = qfx_workspace(ctx, nchunk * m * n * sizeof(float2), &ws);
+  if (st != QFX_OK) return st;
+  float2* part = static_cast<float2*>(ws);
+  const qfx_index_map none{};
+  const qfx_index_map& ar = mapped ? maps[0] : none;
+  const qfx_index_map& ac = mapped ? maps[1] : none;
+  const qfx_index_map& br = mapped ? maps[2] : none;
+  const qfx_index_map& bc = mapped ? maps[3] : none;
+  if (n <= 16) {
+    dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), unsigned(nchunk)};
+    if (mapped)
+      cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, a_mfast, b_nfast, k, part);
+    else
+      cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, false, true, k, part);
+  } else {
+    dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), unsigned(nchunk)};
+    if (mapped)
+      cgemm_smallk_kernel<64, true><<<g, 256, 0, ctx->stream>>>(
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, a_mfast, b_nfast, k, part);
+    else
+      cgemm_smallk_kernel<64, false><<<g, 256, 0, ctx->stream>>>(
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, false, true, k, part);
+  }
+  sum_partials_kernel<<<cap_grid((m * n + 63) / 64), 256, 0, ctx->stream>>>(part, C, nchunk,
+                                                                           m * n);
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
 
 static int launch_gemv_t(qfx_context* ctx, uint64_t m, uint64_t k, const float2* A,
                          const float2* x, float2* y) {
@@ -1368,14 +1420,20 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
-                                                                 {}, {}, {}, {}, false, true);
+                                                                 {}, {}, {}, {}, false, true, k,
+                                                                 nullptr);
     } else {
       dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
       cgemm_smallk_kernel<64, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
-                                                                 {}, {}, {}, {}, false, true);
+                                                                 {}, {}, {}, {}, false, true, k,
+                                                                 nullptr);
     }
     QFX_HIP(hipGetLastError());
     return QFX_OK;
+  }
+  if (precision != QFX_PREC_FP32_EXACT && m > 8 && n > 1 && smallk_splitk_fits(m, n, k)) {
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
+    return launch_smallk_splitk(ctx, A2, B2, C2, m, n, k, false, nullptr, false, true);
   }
   if (precision != QFX_PREC_FP32_EXACT && m <= 8 && n <= 8 && k >= 1024) {
     QfxLaunchScope scope(ctx, QFX_K_GEMV, bytes, flops);
@@ -1511,14 +1569,23 @@ extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c
     if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(
-          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast);
+          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast, k,
+          nullptr);
     } else {
       dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
       cgemm_smallk_kernel<64, true><<<g, 256, 0, ctx->stream>>>(
-          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast);
+          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast, k,
+          nullptr);
     }
     QFX_HIP(hipGetLastError());
     return QFX_OK;
+  }
+  if (precision != QFX_PREC_FP32_EXACT && m > 8 && n > 1 && smallk_splitk_fits(m, n, k)) {
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
+    const qfx_index_map maps[4] = {*a_rows, *a_cols, *b_rows, *b_cols};
+    const bool a_mfast = contiguous(*a_rows, 1) && !contiguous(*a_cols, 3);
+    const bool b_nfast = contiguous(*b_cols, 1) || !contiguous(*b_rows, 1);
+    return launch_smallk_splitk(ctx, A2, B2, C2, m, n, k, true, maps, a_mfast, b_nfast);
   }
   if (precision != QFX_PREC_FP32_EXACT && m <= 8 && n <= 8 && k >= 1024) {
     QfxLaunchScope scope(ctx, QFX_K_GEMV, bytes, flops);

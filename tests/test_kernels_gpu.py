@@ -93,6 +93,8 @@ GEMM_SHAPES = [
     (4096, 512, 16), (1000, 20, 16), (300, 7, 3), (70000, 16, 16), (257, 65, 9),
     # at most 8 x 8 outputs with long k (split-K with register accumulators)
     (1, 8, 4096), (7, 3, 1500), (8, 1, 2048),
+    # small outputs, k a multiple of 16: split-K over the vector-ALU tiles
+    (100, 50, 48), (9, 1000, 64),
     # large tiles 256x128 and 128x256 with ragged edges and odd stage counts
     (32768, 256, 24), (33000, 130, 74), (20000, 600, 36),
 ]
@@ -190,14 +192,14 @@ def test_accumulate_fp64(ctx):
 
 def test_profile_counts_launches(ctx):
     rng = np.random.default_rng(1)
-    a = rand_c64(rng, (256, 256))
-    A, B, C = ctx.upload(a), ctx.upload(a), ctx.alloc(a.nbytes)
+    a = rand_c64(rng, (1024, 512))
+    A, B, C = ctx.upload(a), ctx.upload(a), ctx.alloc(a.nbytes * 2)
     ctx.profile(True)
     ctx.profile_reset()
-    ctx.cgemm(capi.FP32, 256, 256, 256, A, B, C)
-    ctx.reorder(A, C, [256, 256], [1, 0])
+    ctx.cgemm(capi.FP32, 1024, 512, 64, A, B, C)
+    ctx.reorder(A, C, [1024, 512], [1, 0])
     prof = ctx.profile_read()
     ctx.profile(False)
     assert prof["cgemm"]["launches"] == 1 and prof["permute"]["launches"] == 1
-    assert prof["cgemm"]["flops"] == 8.0 * 256 ** 3
+    assert prof["cgemm"]["flops"] == 8.0 * 1024 * 512 * 64
     assert prof["cgemm"]["ms"] > 0
