@@ -73,11 +73,11 @@ def measure_error(qflex, circ, grid, precision):
 
 
 def time_variant(qflex, opts, steps, warmup):
+    """Times `steps` steps with nothing but the work on the stream, then runs
+    one more step with a HIP event pair around every launch (qfx_profile_*),
+    which costs about 8% of a step, for the per-kernel-class figures."""
     for _ in range(warmup):
         qflex.evaluate(opts)
-    qflex.synchronize()
-    qflex.profile_enable(True)
-    qflex.profile_reset()
     qflex.synchronize()
     t0 = time.perf_counter()
     for k in range(steps):
@@ -85,9 +85,18 @@ def time_variant(qflex, opts, steps, warmup):
         log(f"[bench]   step {k + 1}/{steps} done")
     qflex.synchronize()
     dt = time.perf_counter() - t0
+    stats = qflex.last_stats()
+    qflex.profile_enable(True)
+    qflex.profile_reset()
+    qflex.synchronize()
+    t1 = time.perf_counter()
+    qflex.evaluate(opts)
+    qflex.synchronize()
+    dt_prof = time.perf_counter() - t1
     prof = qflex.profile_read()
     qflex.profile_enable(False)
-    return dt, len(r), prof, qflex.last_stats()
+    log("[bench]   profiled step done")
+    return dt, len(r), prof, stats, dt_prof
 
 
 def cpu_baseline(circ, grid):
@@ -188,6 +197,53 @@ def microbench(args):
     return {"microbench": out}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cgemm_traffic_7x7_d32.json")
+MFMA_GEMM_KERNELS = ("cgemm_split_big_kernel", "cgemm_fp16x2_mfma_kernel", "cgemm_f32_mfma_kernel",
+                     "cgemm_bf16_mfma_kernel")
+GEMM_HELPER_KERNELS = ("amax_kernel", "zero_u32_kernel")
+
+
+def traffic_for(variant):
+    """HBM bytes per MFMA cGEMM launch of this variant from the committed
+    rocprofv3 summary (made by --summarize-pmc), or None."""
+    try:
+        t = json.load(open(TRAFFIC_FILE))
+    except (OSError, ValueError):
+        return None
+    v = t.get("variants", {}).get(variant)
+    return v["traffic_bytes_per_launch"] if v else None
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_mib):
+    """HBM bytes per MFMA cGEMM launch from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE, KiB per dispatch), each scaled by what the same
+    counter reads for qfx_copy_kernel moving copy_mib MiB."""
+    import csv
+
+    def load(d, counter):
+        rows = []
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            rows += [r for r in csv.DictReader(open(f)) if r["Counter_Name"] == counter]
+        return rows
+
+    out = {}
+    for name, d, counter in (("fetch", fetch_dir, "FETCH_SIZE"), ("write", write_dir, "WRITE_SIZE")):
+        rows = load(d, counter)
+        copy = [float(r["Counter_Value"]) for r in rows if "qfx_copy_kernel" in r["Kernel_Name"]]
+        gemm = [float(r["Counter_Value"]) for r in rows
+                if any(k in r["Kernel_Name"] for k in MFMA_GEMM_KERNELS)]
+        helper = [float(r["Counter_Value"]) for r in rows
+                  if any(k in r["Kernel_Name"] for k in GEMM_HELPER_KERNELS)]
+        if not copy or not gemm:
+            raise SystemExit(f"{d}: no copy or no cGEMM dispatches with {counter}")
+        factor = (copy_mib << 20) / (max(copy) * 1024.0)
+        out[name] = {"counter": counter, "kib_copy": max(copy), "scale": factor,
+                     "gemm_dispatches": len(gemm),
+                     "bytes_per_launch": factor * 1024.0 * (sum(gemm) + sum(helper)) / len(gemm)}
+    out["traffic_bytes_per_launch"] = out["fetch"]["bytes_per_launch"] + out["write"]["bytes_per_launch"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +256,10 @@ def main():
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
                          "with HIP events through the C-ABI, each --steps times per variant); "
                          "prints one JSON line and skips the workload")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="summarize two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE) "
+                         "of a run with --calibrate-copy-mib into HBM bytes per MFMA cGEMM "
+                         "launch; prints JSON and skips the workload")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="first run one device copy of this many MiB (qfx_copy_kernel), the "
                          "known-size reference for HBM byte counters")
@@ -207,6 +267,11 @@ def main():
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+
+    if args.summarize_pmc:
+        print(json.dumps(summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1],
+                                       args.calibrate_copy_mib or 1024), indent=1))
+        return
 
     from qflex_amd import qflex
 
@@ -228,8 +293,8 @@ def main():
     variants = {}
     for prec in args.variants.split(","):
         log(f"[bench] variant {prec}: warmup {args.warmup}, steps {args.steps}")
-        dt, n_amps, prof, stats = time_variant(qflex, options(circ, order, grid, prec),
-                                               args.steps, args.warmup)
+        dt, n_amps, prof, stats, dt_prof = time_variant(qflex, options(circ, order, grid, prec),
+                                                        args.steps, args.warmup)
         err, err_slices = measure_error(qflex, circ, grid, prec)
         g = prof["cgemm"]
         p = prof["permute"]
@@ -238,9 +303,13 @@ def main():
             "ms_per_step": 1e3 * dt / args.steps,
             "relerr_vs_complex128": err,
             "relerr_slices": err_slices,
-            "cgemm": {"launches": g["launches"], "ms": g["ms"], "tflops": g["flops"] / g["ms"] / 1e9},
-            "permute": {"launches": p["launches"], "ms": p["ms"], "gbps": p["bytes"] / p["ms"] / 1e6},
-            "gemv_ms": prof["gemv"]["ms"], "copy_ms": prof["copy"]["ms"],
+            "profiled_ms_per_step": 1e3 * dt_prof,
+            # per-class HIP-event totals of the profiled step
+            "cgemm": {"launches": g["launches"], "ms": g["ms"],
+                      "tflops": g["flops"] / max(g["ms"], 1e-9) / 1e9},
+            "permute": {"launches": p["launches"], "ms": p["ms"],
+                        "gbps": p["bytes"] / max(p["ms"], 1e-9) / 1e6},
+            "class_ms": {c: round(v["ms"], 3) for c, v in prof.items() if v["launches"]},
             "flops_per_step": stats["flops"], "slices": stats["slices"],
             "_prof": prof,
         }
@@ -257,6 +326,7 @@ def main():
     g = prof["cgemm"]
     achieved = g["flops"] / g["ms"] / 1e9
     p = prof["permute"]
+    p_ms = max(p["ms"], 1e-9)
     out = {
         "metric": METRIC,
         "value": best["amplitudes_per_s"],
@@ -278,13 +348,15 @@ def main():
         "variants": variants,
         "roofline": {"bound": "mfma", "kernel": "cgemm_" + pick,
                      "achieved": achieved, "peak": PEAK_TFLOPS[pick], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS[pick], "traffic": None,
+                     "frac": achieved / PEAK_TFLOPS[pick], "traffic": traffic_for(pick),
+                     "algorithmic_bytes_per_launch": g["bytes"] / max(g["launches"], 1),
                      "avg_launch_ms": g["ms"] / max(g["launches"], 1),
                      "flops_per_launch": g["flops"] / max(g["launches"], 1)},
-        "roofline_permute": {"bound": "hbm", "achieved": p["bytes"] / p["ms"] / 1e6,
+        "roofline_permute": {"bound": "hbm", "achieved": p["bytes"] / p_ms / 1e6,
                              "peak": HBM_GBPS, "unit": "GB/s",
-                             "frac": p["bytes"] / p["ms"] / 1e6 / HBM_GBPS,
-                             "avg_launch_ms": p["ms"] / max(p["launches"], 1)},
+                             "frac": p["bytes"] / p_ms / 1e6 / HBM_GBPS,
+                             "avg_launch_ms": p["ms"] / max(p["launches"], 1),
+                             "launches_per_step": p["launches"]},
     }
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline (bounded sample)")

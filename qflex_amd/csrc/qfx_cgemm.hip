@@ -316,39 +316,98 @@ __global__ __launch_bounds__(256) void cgemm_bf16_mfma_kernel(const float2* __re
 }
 
 // ------------------------------------------------------------ split-FP16
-// max(|re|, |im|) over n complex values, as float bits (order-free: the
-// result is the same whatever the reduction order).  One atomic per block;
-// *out must start at 0.
-__global__ __launch_bounds__(256) void amax_kernel(const float2* __restrict__ x, uint64_t n,
-                                                   unsigned int* __restrict__ out) {
+// Partial maxima of max(|re|, |im|) over the elements of A (blocks
+// [0, blocks_a)) and of B (the remaining blocks), one float per block, in a
+// single launch; the GEMM blocks take the maximum of the partials.  The
+// result does not depend on the order of the reduction.
+__device__ __forceinline__ float absmax4(float4 v) {
+  return fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+}
+
+__global__ __launch_bounds__(256) void amax_partial_kernel(const float2* __restrict__ A,
+                                                           uint64_t na,
+                                                           const float2* __restrict__ B,
+                                                           uint64_t nb, int blocks_a,
+                                                           float* __restrict__ part) {
   __shared__ float red[4];
+  const bool is_a = int(blockIdx.x) < blocks_a;
+  const float2* x = is_a ? A : B;
+  const uint64_t n = is_a ? na : nb;
+  const uint64_t nblk = is_a ? uint64_t(blocks_a) : uint64_t(gridDim.x - blocks_a);
+  const uint64_t b = is_a ? blockIdx.x : blockIdx.x - blocks_a;
+  const uint64_t stride = nblk * 256;
   float m = 0.f;
-  for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-       i += uint64_t(gridDim.x) * blockDim.x) {
-    float2 v = x[i];
-    m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+  if ((reinterpret_cast<uintptr_t>(x) & 15) == 0) {
+    // pairs of complex values as float4, four loads in flight per thread
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const uint64_t n4 = n / 2;
+    uint64_t i = b * 256 + threadIdx.x;
+    for (; i + 3 * stride < n4; i += 4 * stride) {
+      const float4 v0 = x4[i], v1 = x4[i + stride], v2 = x4[i + 2 * stride],
+                   v3 = x4[i + 3 * stride];
+      m = fmaxf(m, fmaxf(fmaxf(absmax4(v0), absmax4(v1)), fmaxf(absmax4(v2), absmax4(v3))));
+    }
+    for (; i < n4; i += stride) m = fmaxf(m, absmax4(x4[i]));
+    if ((n & 1) && b == 0 && threadIdx.x == 0) {
+      const float2 v = x[n - 1];
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+  } else {
+    for (uint64_t i = b * 256 + threadIdx.x; i < n; i += stride) {
+      const float2 v = x[i];
+      m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    if (m > 0.f) atomicMax(out, __float_as_uint(m));
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
-__global__ void zero_u32_kernel(unsigned int* p, int n) {
-  if (int(threadIdx.x) < n) p[threadIdx.x] = 0u;
-}
+// Partial maxima of the operands of one split-FP16 GEMM.
+struct AmaxRef {
+  const float* part;  // na partials of A, then nb of B
+  int na, nb;
+};
 
 // Power of two s with max|x| * s in [2^14, 2^15) (FP16 max is 65504).
-__device__ __forceinline__ float fp16_scale(unsigned int amax_bits) {
-  const float amax = __uint_as_float(amax_bits);
+__device__ __forceinline__ float fp16_scale(float amax) {
   if (!(amax > 0.f)) return 1.f;
   int e;
   frexpf(amax, &e);  // amax = f * 2^e, f in [0.5, 1)
   return ldexpf(1.f, 15 - e);
+}
+
+// Operand scales from the partial maxima, for every thread of a block of
+// THREADS threads; red holds 2 * THREADS / 64 floats of LDS that no other
+// thread touches until the block's next barrier.
+template <int THREADS>
+__device__ __forceinline__ void operand_scales(const AmaxRef& r, float* red, float& sa,
+                                               float& sb) {
+  constexpr int W = THREADS / 64;
+  float ma = 0.f, mb = 0.f;
+  for (int i = threadIdx.x; i < r.na; i += THREADS) ma = fmaxf(ma, r.part[i]);
+  for (int i = threadIdx.x; i < r.nb; i += THREADS) mb = fmaxf(mb, r.part[r.na + i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    ma = fmaxf(ma, __shfl_xor(ma, off));
+    mb = fmaxf(mb, __shfl_xor(mb, off));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[threadIdx.x >> 6] = ma;
+    red[W + (threadIdx.x >> 6)] = mb;
+  }
+  __syncthreads();
+  ma = red[0];
+  mb = red[W];
+#pragma unroll
+  for (int w = 1; w < W; ++w) {
+    ma = fmaxf(ma, red[w]);
+    mb = fmaxf(mb, red[W + w]);
+  }
+  sa = fp16_scale(ma);
+  sb = fp16_scale(mb);
 }
 
 // x = hi + lo with hi = fp16(x) and lo = fp16(x - hi): 22 significant bits
@@ -368,9 +427,10 @@ typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
 // unscaled on output.
 __global__ __launch_bounds__(256) void cgemm_fp16x2_mfma_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
-    int N, int K, const unsigned int* __restrict__ amax) {
+    int N, int K, AmaxRef amax) {
   __shared__ __align__(16) _Float16 As[2][kBM * kRowB];
   __shared__ __align__(16) _Float16 Bs[2][2 * kBN * kRowB];
+  __shared__ float red[8];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -379,8 +439,8 @@ __global__ __launch_bounds__(256) void cgemm_fp16x2_mfma_kernel(
   const int wn = wave & 1;
   const int m0 = blockIdx.x * kBM;
   const int n0 = blockIdx.y * kBN;
-  const float sa = fp16_scale(amax[0]);
-  const float sb = fp16_scale(amax[1]);
+  float sa, sb;
+  operand_scales<256>(amax, red, sa, sb);
   const float unscale = 1.f / (sa * sb);
 
   float2 ra[8], rb[4];
@@ -576,7 +636,7 @@ __device__ __forceinline__ uint64_t map_offset(const qfx_index_map& d, uint64_t 
 template <int MODE, int WM, int WN, int J, int AM>
 __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
-    int N, int K, const unsigned int* __restrict__ amax, qfx_index_map a_rows,
+    int N, int K, AmaxRef amax, qfx_index_map a_rows,
     qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols) {
   static_assert(WM * WN == 8, "8 waves");
   typedef typename SplitTraits<MODE>::T T;
@@ -600,10 +660,6 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
   const int m0 = blockIdx.x * kBM;
   const int n0 = blockIdx.y * kBN;
   float sa = 1.f, sb = 1.f;
-  if constexpr (MODE == 2) {
-    sa = fp16_scale(amax[0]);
-    sb = fp16_scale(amax[1]);
-  }
 
   // global -> register assignment (f = tid + 512*h, g = tid + 512*h)
   // A, AM 0/1: row = f >> 2, q = f & 3 (complex k = 2q, 2q+1 of the stage)
@@ -789,6 +845,10 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
 
   float4 ra0[kAH], rb0[kBH], ra1[kAH], rb1[kBH];
   load_tile(0, ra0, rb0);
+  // the scales are reduced while the first loads are in flight (in the
+  // second stage buffer, first written after the prologue's barrier)
+  if constexpr (MODE == 2)
+    operand_scales<512>(amax, reinterpret_cast<float*>(smem + kBuf), sa, sb);
   store_tile(0, ra0, rb0);
   load_tile(nkt > 1 ? 1 : 0, ra1, rb1);
   __syncthreads();
@@ -849,7 +909,7 @@ hipError_t split_big_prepare() {
 
 template <int MODE, int WM, int WN, int J, int AM>
 hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
-                            uint64_t m, uint64_t n, uint64_t k, const unsigned int* amax,
+                            uint64_t m, uint64_t n, uint64_t k, AmaxRef amax,
                             const qfx_index_map* maps) {
   hipError_t e = split_big_prepare<MODE, WM, WN, J, AM>();
   if (e != hipSuccess) return e;
@@ -881,7 +941,7 @@ BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
 template <int MODE, int AM>
 hipError_t launch_split_big_layout(BigLayout lay, hipStream_t stream, const float2* A,
                                    const float2* B, float2* C, uint64_t m, uint64_t n,
-                                   uint64_t k, const unsigned int* amax,
+                                   uint64_t k, AmaxRef amax,
                                    const qfx_index_map* maps) {
   if constexpr (MODE == 0) {
     return launch_split_big<0, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
@@ -905,7 +965,7 @@ bool split_big_mapped_fits(int precision, BigLayout lay, uint64_t k) {
 template <int AM>
 hipError_t launch_split_big_prec(int precision, BigLayout lay, hipStream_t stream,
                                  const float2* A, const float2* B, float2* C, uint64_t m,
-                                 uint64_t n, uint64_t k, const unsigned int* amax,
+                                 uint64_t n, uint64_t k, AmaxRef amax,
                                  const qfx_index_map* maps) {
   if (precision == QFX_PREC_SPLIT_BF16X3)
     return launch_split_big_layout<0, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
@@ -981,6 +1041,31 @@ __global__ __launch_bounds__(256) void gemv_t_partial_kernel(const float2* __res
     }
     __syncthreads();
   }
+}
+
+// Stage 2 for few columns and many partials: one block per column, each
+// thread sums a fixed strided subset, then a fixed LDS tree.
+__global__ __launch_bounds__(256) void sum_partials_narrow_kernel(const float2* __restrict__ part,
+                                                                  float2* __restrict__ y,
+                                                                  uint64_t nsplit, uint64_t K) {
+  __shared__ float2 red[256];
+  const uint64_t c = blockIdx.x;
+  float2 acc = make_float2(0.f, 0.f);
+  for (uint64_t s = threadIdx.x; s < nsplit; s += 256) {
+    const float2 v = part[s * K + c];
+    acc.x += v.x;
+    acc.y += v.y;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+      red[threadIdx.x].x += red[threadIdx.x + w].x;
+      red[threadIdx.x].y += red[threadIdx.x + w].y;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) y[c] = red[0];
 }
 
 // Stage 2: y[c] = sum_s part[s][c]; 64 columns x 4 split groups per block,
@@ -1382,19 +1467,19 @@ extern "C" int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, con
   return QFX_OK;
 }
 
-// Operand scales of the split-FP16 form: max|A| and max|B| into a 2-word
-// workspace on the device (no host round trip).
+// Operand scales of the split-FP16 form: partial maxima of |A| and |B| in
+// one launch (at most 256 blocks per operand, at least 16 Ki elements
+// each); the GEMM kernel reduces them (no host round trip, no atomics).
 static int launch_amax(qfx_context* ctx, const float2* A, uint64_t na, const float2* B,
-                       uint64_t nb, unsigned int** amax) {
+                       uint64_t nb, AmaxRef* ref) {
+  const int ba = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (na + 16383) / 16384)));
+  const int bb = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (nb + 16383) / 16384)));
   void* ws = nullptr;
-  int st = qfx_workspace(ctx, 16, &ws);
+  int st = qfx_workspace(ctx, size_t(ba + bb) * sizeof(float), &ws);
   if (st != QFX_OK) return st;
-  *amax = static_cast<unsigned int*>(ws);
-  zero_u32_kernel<<<1, 64, 0, ctx->stream>>>(*amax, 2);
-  amax_kernel<<<cap_grid(std::min<uint64_t>((na + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
-      A, na, *amax);
-  amax_kernel<<<cap_grid(std::min<uint64_t>((nb + 1023) / 1024, 512)), 256, 0, ctx->stream>>>(
-      B, nb, *amax + 1);
+  float* part = static_cast<float*>(ws);
+  amax_partial_kernel<<<unsigned(ba + bb), 256, 0, ctx->stream>>>(A, na, B, nb, ba, part);
+  *ref = AmaxRef{part, ba, bb};
   return QFX_OK;
 }
 
@@ -1444,7 +1529,7 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     float2* part = static_cast<float2*>(ws);
     cgemm_tiny_mn_partial_kernel<<<unsigned(nchunk), 256, 0, ctx->stream>>>(A2, B2, part, int(m),
                                                                             int(n), k);
-    sum_partials_kernel<<<1, 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
+    sum_partials_narrow_kernel<<<unsigned(m * n), 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
     QFX_HIP(hipGetLastError());
     return QFX_OK;
   }
@@ -1495,7 +1580,7 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
        (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0))
           ? choose_big_layout(precision, m, n)
           : kLayoutNone;
-  unsigned int* amax = nullptr;
+  AmaxRef amax{nullptr, 0, 0};
   if (precision == QFX_PREC_SPLIT_FP16) {
     int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax);
     if (st != QFX_OK) return st;
@@ -1596,7 +1681,7 @@ extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c
     float2* part = static_cast<float2*>(ws);
     cgemm_tiny_mn_mapped_partial_kernel<<<unsigned(nchunk), 256, 0, ctx->stream>>>(
         A2, B2, part, int(m), int(n), k, *a_rows, *a_cols, *b_rows, *b_cols);
-    sum_partials_kernel<<<1, 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
+    sum_partials_narrow_kernel<<<unsigned(m * n), 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
     QFX_HIP(hipGetLastError());
     return QFX_OK;
   }
@@ -1612,7 +1697,7 @@ extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c
       !split_big_mapped_fits(precision, layout, k))
     return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_mapped_c64: no kernel reads this form");
   QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
-  unsigned int* amax = nullptr;
+  AmaxRef amax{nullptr, 0, 0};
   if (precision == QFX_PREC_SPLIT_FP16) {
     int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax);
     if (st != QFX_OK) return st;
