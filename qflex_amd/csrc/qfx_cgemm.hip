@@ -22,6 +22,7 @@
 namespace {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 // ------------------------------------------------------------ scalar path
@@ -553,7 +554,9 @@ __global__ __launch_bounds__(256) void cgemm_fp16x2_mfma_kernel(
 // 64*WM complex rows x 64*WN complex columns.  One stage = 8 complex k = 16
 // reals = one 32x32x16 MFMA k-step.  LDS holds two stages of NT planes:
 //   As[buf][t][row][16 reals + 8 pad]      (48-byte rows: conflict-free b128)
-//   Bs[buf][t][part*BN + n][16 + 8 pad]    Bhat columns, as in the kernel above
+//   Bs[buf][t][2*n + part][16 + 8 pad]     Bhat columns in Bhat order (the two
+//                                          columns of complex n adjacent, so
+//                                          that a fragment read is conflict-free)
 // Software pipeline, one barrier per stage: while the MFMAs of stage s run
 // from LDS buffer s&1, the raw operands of stage s+1 (loaded into registers
 // during stage s-1) are split and stored into the other buffer, and the
@@ -693,40 +696,52 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
     }
     __syncthreads();
   }
+  // Operands are read through buffer resources: an element outside the
+  // tile's range gets an offset past the end of the buffer and reads as 0,
+  // so there is no branch and no select near the loads (the waits for a
+  // stage's operands then never include the next stage's loads).
+  const __amdgpu_buffer_rsrc_t rsrc_a = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float2*>(A), 0, int(uint64_t(M) * K * sizeof(float2)), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsrc_b = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float2*>(B), 0, int(uint64_t(K) * N * sizeof(float2)), 0x00020000);
+  constexpr unsigned kOutside = 0xfffffff0u;
+  auto load16 = [](__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+    const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, int(byte_off), 0, 0);
+    return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
+                       __uint_as_float(v.w));
+  };
   auto load_tile = [&](int kt, float4* ra, float4* rb) {
     const int k0 = kt * kBBK;
-    uint64_t a_kbase = 0, b_kbase = 0;
+    unsigned a_kbase = 0, b_kbase = 0;
     if constexpr (AM != 0) {
-      a_kbase = ktab[2 * kt];
-      b_kbase = ktab[2 * kt + 1];
+      a_kbase = unsigned(ktab[2 * kt]);
+      b_kbase = unsigned(ktab[2 * kt + 1]);
     }
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + 512 * h;
+      int gm, gk;
+      unsigned off;
       if constexpr (AM == 2) {
-        const int row = 2 * (f >> 3), kk = f & 7;
-        const int gm = m0 + row, gk = k0 + kk;
-        ra[h] = (gm < M && gk < K)
-                    ? *reinterpret_cast<const float4*>(A + a_off[h] + a_kbase + a_coff)
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
+        gm = m0 + 2 * (f >> 3);
+        gk = k0 + (f & 7);
+        off = unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
       } else {
-        const int row = f >> 2, q = f & 3;
-        const int gm = m0 + row, gk = k0 + 2 * q;
-        const float2* src = AM == 0 ? A + uint64_t(gm) * K + gk : A + a_off[h] + a_kbase + a_coff;
-        ra[h] = (gm < M && gk < K) ? *reinterpret_cast<const float4*>(src)
-                                   : make_float4(0.f, 0.f, 0.f, 0.f);
+        gm = m0 + (f >> 2);
+        gk = k0 + 2 * (f & 3);
+        off = AM == 0 ? unsigned(gm) * unsigned(K) + unsigned(gk)
+                      : unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
       }
+      ra[h] = load16(rsrc_a, (gm < M && gk < K) ? off * unsigned(sizeof(float2)) : kOutside);
     }
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + 512 * h;
-      const int kk = g & 7, nq = g >> 3;
-      const int gk = k0 + kk, gn = n0 + 2 * nq;
-      const float2* src =
-          AM == 0 ? B + uint64_t(gk) * N + gn : B + b_kbase + b_roff + b_off[h];
-      rb[h] = ((kBT % 512 == 0 || g < kBT) && gk < K && gn < N)
-                  ? *reinterpret_cast<const float4*>(src)
-                  : make_float4(0.f, 0.f, 0.f, 0.f);
+      const int gk = k0 + (g & 7), gn = n0 + 2 * (g >> 3);
+      const unsigned off = AM == 0 ? unsigned(gk) * unsigned(N) + unsigned(gn)
+                                   : b_kbase + unsigned(b_roff) + unsigned(b_off[h]);
+      const bool in = (kBT % 512 == 0 || g < kBT) && gk < K && gn < N;
+      rb[h] = load16(rsrc_b, in ? off * unsigned(sizeof(float2)) : kOutside);
     }
   };
   typedef T T2 __attribute__((ext_vector_type(2)));
@@ -778,8 +793,8 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
           const T br = t3[2 * c][t], bi = t3[2 * c + 1][t];
           T2 p0 = {br, static_cast<T>(-static_cast<float>(bi))};
           T2 p1 = {bi, br};
-          *reinterpret_cast<T2*>(bb + t * kBPlane + n * kRowL + 2 * kk) = p0;
-          *reinterpret_cast<T2*>(bb + t * kBPlane + (kBN + n) * kRowL + 2 * kk) = p1;
+          *reinterpret_cast<T2*>(bb + t * kBPlane + (2 * n) * kRowL + 2 * kk) = p0;
+          *reinterpret_cast<T2*>(bb + t * kBPlane + (2 * n + 1) * kRowL + 2 * kk) = p1;
         }
       }
     }
@@ -810,8 +825,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const int rc = wn * 32 * J + j * 32 + col;  // real column of Bhat in the block
-        const int row = (rc & 1) * kBN + (rc >> 1);
-        b[t][j] = *reinterpret_cast<const V*>(bb + t * kBPlane + row * kRowL + 8 * half);
+        b[t][j] = *reinterpret_cast<const V*>(bb + t * kBPlane + rc * kRowL + 8 * half);
       }
     }
 #pragma unroll
@@ -832,7 +846,6 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
   // pairs around groups of VALU (operand split) and one LDS store.
   auto schedule = [&]() {
     constexpr int kMfma = 2 * J * SplitTraits<MODE>::PRODUCTS;
-    __builtin_amdgcn_sched_group_barrier(0x020, kAH + kBH, 0);  // VMEM read
     __builtin_amdgcn_sched_group_barrier(0x100, (2 + J) * NT, 0);  // DS read
 #pragma unroll
     for (int i = 0; i < kMfma / 2; ++i) {
@@ -852,22 +865,30 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
   store_tile(0, ra0, rb0);
   load_tile(nkt > 1 ? 1 : 0, ra1, rb1);
   __syncthreads();
-  for (int kt = 0; kt < nkt; kt += 2) {
-    // buffer 0 holds stage kt, registers 1 hold stage kt+1
+  // Pairs of stages, both unconditional (a conditional second stage would
+  // let the compiler sink the first stage's loads into it); an odd last
+  // stage runs after the loop.
+  const int npairs = nkt / 2;
+  for (int p = 0; p < npairs; ++p) {
+    const int kt = 2 * p;
+    // buffer 0 holds stage kt, registers 1 hold stage kt+1.  The loads stay
+    // at the top of the stage (sched_barrier): a full stage of latency
+    // hiding before this register set is consumed.
     load_tile(min(kt + 2, nkt - 1), ra0, rb0);
+    __builtin_amdgcn_sched_barrier(0);
     compute(0);
     store_tile(1, ra1, rb1);
     schedule();
     __syncthreads();
-    if (kt + 1 < nkt) {
-      // buffer 1 holds stage kt+1, registers 0 hold stage kt+2
-      load_tile(min(kt + 3, nkt - 1), ra1, rb1);
-      compute(1);
-      store_tile(0, ra0, rb0);
-      schedule();
-      __syncthreads();
-    }
+    // buffer 1 holds stage kt+1, registers 0 hold stage kt+2
+    load_tile(min(kt + 3, nkt - 1), ra1, rb1);
+    __builtin_amdgcn_sched_barrier(0);
+    compute(1);
+    store_tile(0, ra0, rb0);
+    schedule();
+    __syncthreads();
   }
+  if (nkt & 1) compute(0);  // stage nkt-1
 
   const float unscale = 1.f / (sa * sb);
   float* Cf = reinterpret_cast<float*>(C);
@@ -927,6 +948,13 @@ hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B
 // per wave): 4x2 (256 x 128 complex), 2x4 (128 x 256), 8x1 with J = 2
 // (512 x 32, for n < 128).
 enum BigLayout { kLayoutNone = 0, kLayout4x2, kLayout2x4, kLayout8x1 };
+
+// The large-tile kernel addresses each operand with 32-bit byte offsets
+// into a buffer resource.
+bool big_operands_fit(uint64_t m, uint64_t n, uint64_t k) {
+  const uint64_t lim = (uint64_t(1) << 31) - 64;
+  return m * k * sizeof(float2) <= lim && k * n * sizeof(float2) <= lim;
+}
 
 BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
   auto blocks = [&](uint64_t bm, uint64_t bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
@@ -1576,7 +1604,7 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   // Large tiles (one block per CU) when there are enough of them to fill
   // the chip; float4 operand loads need even k and n and 16-byte alignment.
   const BigLayout layout =
-      (precision != QFX_PREC_FP32 && (k % 2 == 0) && (n % 2 == 0) &&
+      (precision != QFX_PREC_FP32 && (k % 2 == 0) && (n % 2 == 0) && big_operands_fit(m, n, k) &&
        (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0))
           ? choose_big_layout(precision, m, n)
           : kLayoutNone;
@@ -1688,7 +1716,7 @@ extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c
   const bool split = precision == QFX_PREC_SPLIT_FP16 || precision == QFX_PREC_SPLIT_BF16 ||
                      precision == QFX_PREC_SPLIT_BF16X3;
   const BigLayout layout =
-      (split && m <= 0x7fffffffULL && n <= 0x7fffffffULL && k >= 8 && k <= 0x7fffffffULL &&
+      (split && big_operands_fit(m, n, k) && k >= 8 &&
        reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0)
           ? choose_big_layout(precision, m, n)
           : kLayoutNone;
