@@ -16,6 +16,7 @@
 // instruction).  SPLIT_BF16 keeps hi*hi + hi*mid + mid*hi (3 products),
 // SPLIT_BF16X3 adds mid*mid + hi*lo + lo*hi (6 products).
 #include <algorithm>
+#include <type_traits>
 
 #include "qfx_internal.h"
 
@@ -856,39 +857,38 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
     }
   };
 
-  float4 ra0[kAH], rb0[kBH], ra1[kAH], rb1[kBH];
-  load_tile(0, ra0, rb0);
+  // Two register sets: stage s is loaded into set s & 1 at stage s - 2 and
+  // split into LDS buffer s & 1 at stage s - 1.  (A third set, hiding two
+  // stages of latency, measured slower: the kernel is not latency-bound.)
+  float4 ra[2][kAH], rb[2][kBH];
+  load_tile(0, ra[0], rb[0]);
   // the scales are reduced while the first loads are in flight (in the
   // second stage buffer, first written after the prologue's barrier)
   if constexpr (MODE == 2)
     operand_scales<512>(amax, reinterpret_cast<float*>(smem + kBuf), sa, sb);
-  store_tile(0, ra0, rb0);
-  load_tile(nkt > 1 ? 1 : 0, ra1, rb1);
+  load_tile(min(1, nkt - 1), ra[1], rb[1]);
+  store_tile(0, ra[0], rb[0]);
   __syncthreads();
-  // Pairs of stages, both unconditional (a conditional second stage would
-  // let the compiler sink the first stage's loads into it); an odd last
-  // stage runs after the loop.
-  const int npairs = nkt / 2;
-  for (int p = 0; p < npairs; ++p) {
-    const int kt = 2 * p;
-    // buffer 0 holds stage kt, registers 1 hold stage kt+1.  The loads stay
-    // at the top of the stage (sched_barrier): a full stage of latency
-    // hiding before this register set is consumed.
-    load_tile(min(kt + 2, nkt - 1), ra0, rb0);
+  // One stage: the loads of stage s + 2 stay at its top (sched_barrier),
+  // then the MFMAs of stage s interleaved with the split of stage s + 1.
+  // The main loop runs pairs of stages unconditionally (a conditional stage
+  // would let the compiler sink earlier loads into it); an odd last stage
+  // follows it.
+  auto stage = [&](auto si, int s) {
+    constexpr int SI = decltype(si)::value;
+    load_tile(min(s + 2, nkt - 1), ra[SI], rb[SI]);
     __builtin_amdgcn_sched_barrier(0);
-    compute(0);
-    store_tile(1, ra1, rb1);
+    compute(SI);
+    store_tile(SI ^ 1, ra[SI ^ 1], rb[SI ^ 1]);
     schedule();
     __syncthreads();
-    // buffer 1 holds stage kt+1, registers 0 hold stage kt+2
-    load_tile(min(kt + 3, nkt - 1), ra1, rb1);
-    __builtin_amdgcn_sched_barrier(0);
-    compute(1);
-    store_tile(0, ra0, rb0);
-    schedule();
-    __syncthreads();
+  };
+  int s0 = 0;
+  for (; s0 + 2 <= nkt; s0 += 2) {
+    stage(std::integral_constant<int, 0>{}, s0);
+    stage(std::integral_constant<int, 1>{}, s0 + 1);
   }
-  if (nkt & 1) compute(0);  // stage nkt-1
+  if (s0 < nkt) compute(0);  // stage nkt - 1
 
   const float unscale = 1.f / (sa * sb);
   float* Cf = reinterpret_cast<float*>(C);
@@ -958,8 +958,11 @@ bool big_operands_fit(uint64_t m, uint64_t n, uint64_t k) {
 
 BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
   auto blocks = [&](uint64_t bm, uint64_t bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
-  if (n >= 512 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(128, 256) >= 128) return kLayout2x4;
+  // 128 x 256 measured faster for tall operands (m >= 128 n), 256 x 128 otherwise
+  if (n >= 512 && m >= 128 * n && precision != QFX_PREC_SPLIT_BF16X3 && blocks(128, 256) >= 128)
+    return kLayout2x4;
   if (n >= 128 && blocks(256, 128) >= 128) return kLayout4x2;
+  if (n >= 512 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(128, 256) >= 128) return kLayout2x4;
   // three BF16 planes of the 512-row tile would exceed the 160 KiB of LDS
   if (n >= 32 && n < 128 && precision != QFX_PREC_SPLIT_BF16X3 && blocks(512, 32) >= 128)
     return kLayout8x1;
