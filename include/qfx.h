@@ -163,6 +163,29 @@ int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, const qfx_c64*
 int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n, qfx_c128* amps,
                         uint64_t offset);
 
+/* ---- the cost model's network ------------------------------------------ */
+/* No counterpart in the reference: the MLP that scores candidate contraction
+ * orderings (features from the engine's symbolic plan, labels from its own
+ * timings).  64 -> 256 -> 256 -> 1 with ReLU, BF16 operands (round to nearest
+ * even) and FP32 accumulation.  params: one FP32 buffer of QFX_MLP_PARAMS
+ * values: W1[64][256], b1[256], W2[256][256], b2[256], w3[256], b3[1]
+ * (W[i][j] maps input i to output j).  X is [batch][64] FP32, row-major. */
+#define QFX_MLP_IN 64
+#define QFX_MLP_HIDDEN 256
+#define QFX_MLP_PARAMS 82689
+
+/* y[b] = network(X[b]), b < batch (batch <= 2^20). */
+int qfx_mlp_forward(qfx_context* ctx, uint64_t batch, const float* X, const float* params,
+                    float* y);
+/* Forward + backward of the mean squared error against target[batch]:
+ * grads (QFX_MLP_PARAMS FP32, same layout as params) and *loss (one FP32 in
+ * device memory) = mean (y - target)^2. */
+int qfx_mlp_backward(qfx_context* ctx, uint64_t batch, const float* X, const float* target,
+                     const float* params, float* grads, float* loss);
+/* One Adam update of params (FP32 master copy) with moments m, v; step >= 1. */
+int qfx_mlp_adam(qfx_context* ctx, float* params, const float* grads, float* m, float* v, int step,
+                 float lr, float beta1, float beta2, float eps);
+
 /* ---- timing ------------------------------------------------------------ */
 
 /* When enabled, every kernel the context launches is bracketed by a pair of

@@ -15,6 +15,7 @@ PRECISIONS = {"fp32": FP32, "split_bf16": SPLIT_BF16, "fp32_exact": FP32_EXACT,
               "split_bf16x3": SPLIT_BF16X3, "split_fp16": SPLIT_FP16}
 K_PERMUTE, K_CGEMM, K_GEMV, K_DOT, K_ACCUMULATE, K_COPY, K_CGEMM_SMALL = range(7)
 KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy", "cgemm_small"]
+MLP_IN, MLP_HIDDEN, MLP_PARAMS = 64, 256, 82689
 
 _c = ctypes
 _vp = _c.c_void_p
@@ -66,6 +67,10 @@ SIGNATURES = {
     "qfx_cgemv_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _vp, _vp, _vp]),
     "qfx_cdotu_c64": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
     "qfx_accumulate_c128": (_c.c_int, [_vp, _vp, _u64, _vp, _u64]),
+    "qfx_mlp_forward": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
+    "qfx_mlp_backward": (_c.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp]),
+    "qfx_mlp_adam": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_int, _c.c_float, _c.c_float,
+                                _c.c_float, _c.c_float]),
     "qfx_profile_enable": (_c.c_int, [_vp, _c.c_int]),
     "qfx_profile_reset": (_c.c_int, [_vp]),
     "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
@@ -166,6 +171,18 @@ class Context:
             return False
         check(st)
         return True
+
+    # ---- the cost model's network (QFX_MLP_PARAMS FP32 parameters)
+    def mlp_forward(self, batch, X, params, y):
+        check(load().qfx_mlp_forward(self.handle, batch, X.ptr, params.ptr, y.ptr))
+
+    def mlp_backward(self, batch, X, target, params, grads, loss):
+        check(load().qfx_mlp_backward(self.handle, batch, X.ptr, target.ptr, params.ptr,
+                                      grads.ptr, loss.ptr))
+
+    def mlp_adam(self, params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        check(load().qfx_mlp_adam(self.handle, params.ptr, grads.ptr, m.ptr, v.ptr, step, lr,
+                                  beta1, beta2, eps))
 
     def cgemv(self, trans, m, k, A, x, y):
         check(load().qfx_cgemv_c64(self.handle, trans, m, k, A.ptr, x.ptr, y.ptr))

@@ -5,6 +5,7 @@
 #include "contraction_utils.h"
 
 #include <algorithm>
+#include <chrono>
 #include <iostream>
 #include <regex>
 #include <sstream>
@@ -207,6 +208,101 @@ void ValidateOrdering(const std::list<ContractionOperation>& ordering) {
 
 // ------------------------------------------------------------ planning
 
+SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
+                              const std::vector<std::vector<Tensor>>& tensor_grid) {
+  SymbolicPlan plan;
+  std::unordered_map<std::string, std::size_t> index_dimension;
+  std::vector<std::vector<std::vector<std::string>>> grid_indices(tensor_grid.size());
+  for (std::size_t i = 0; i < tensor_grid.size(); ++i) {
+    grid_indices[i].resize(tensor_grid[i].size());
+    for (std::size_t j = 0; j < tensor_grid[i].size(); ++j) {
+      const Tensor& t = tensor_grid[i][j];
+      grid_indices[i][j] = t.get_indices();
+      for (const auto& index : t.get_indices())
+        index_dimension[index] = t.get_index_to_dimension().at(index);
+      if (t.data()) plan.max_grid = std::max(plan.max_grid, t.size());
+    }
+  }
+
+  // Bond cuts remove their index (and multiply the work after them by
+  // their value count); terminal cuts keep the output index open with one
+  // entry per requested value.
+  std::unordered_map<std::string, std::vector<std::string>> patch_indices;
+  std::size_t multiplicity = 1;
+  auto sym_diff = [](const std::vector<std::string>& a, const std::vector<std::string>& b) {
+    return _vector_subtraction(_vector_union(a, b), _vector_intersection(a, b));
+  };
+  auto size_of = [&](const std::vector<std::string>& idx) {
+    std::size_t s = 1;
+    for (const auto& x : idx) s *= index_dimension.at(x);
+    return s;
+  };
+  // C = A . B as contract_pair executes it: left = A \ B, common = A n B,
+  // right = B \ A.
+  auto record = [&](int op, const std::vector<std::string>& a, const std::vector<std::string>& b) {
+    if (a.empty() || b.empty()) return;
+    ContractionStep st;
+    st.op = op;
+    st.m = size_of(_vector_subtraction(a, b));
+    st.k = size_of(_vector_intersection(a, b));
+    st.n = size_of(_vector_subtraction(b, a));
+    st.multiplicity = multiplicity;
+    const auto c = sym_diff(a, b);
+    st.out_rank = c.size();
+    st.out_size = size_of(c);
+    plan.steps.push_back(st);
+  };
+  for (const auto& op : ordering) {
+    switch (op.op_type) {
+      case ContractionOperation::EXPAND: {
+        auto& pi = patch_indices[op.expand.id];
+        const auto& site = grid_indices.at(op.expand.tensor[0]).at(op.expand.tensor[1]);
+        record(ContractionOperation::EXPAND, pi, site);
+        pi = sym_diff(pi, site);
+        auto& mx = plan.patch_max_size[op.expand.id];
+        mx = std::max(mx, size_of(pi));
+        break;
+      }
+      case ContractionOperation::CUT: {
+        try {
+          const std::string cut_index = index_name(op.cut.tensors);
+          const std::size_t dim = index_dimension.count(cut_index) ? index_dimension[cut_index] : 1;
+          const std::size_t count = op.cut.values.empty() ? dim : op.cut.values.size();
+          int side = 0;
+          for (const auto& tensor : op.cut.tensors) {
+            auto& indices = grid_indices.at(tensor[0]).at(tensor[1]);
+            const std::size_t full = tensor_grid[tensor[0]][tensor[1]].size();
+            if (op.cut.tensors.size() > 1) {
+              indices = _vector_subtraction(indices, {cut_index});
+              plan.side_buffers[ContractionData::cut_copy_name(op.cut.tensors, side)] =
+                  full / std::max<std::size_t>(dim, 1);
+            } else {
+              index_dimension[cut_index] = count;
+              plan.side_buffers[ContractionData::cut_copy_name(op.cut.tensors, side)] =
+                  full / std::max<std::size_t>(dim, 1) * count;
+            }
+            ++side;
+          }
+          if (op.cut.tensors.size() > 1) multiplicity *= count;
+        } catch (const std::string& err_msg) {
+          throw ERROR_MSG("Failed during CUT. Error:\n\t[", err_msg, "]");
+        }
+        break;
+      }
+      case ContractionOperation::MERGE: {
+        auto& pi = patch_indices[op.merge.target_id];
+        const auto src = patch_indices[op.merge.source_id];
+        record(ContractionOperation::MERGE, src, pi);
+        pi = sym_diff(pi, src);
+        auto& mx = plan.patch_max_size[op.merge.target_id];
+        mx = std::max(mx, size_of(pi));
+        break;
+      }
+    }
+  }
+  return plan;
+}
+
 ContractionData ContractionData::Initialize(const std::list<ContractionOperation>& ordering,
                                             std::vector<std::vector<Tensor>>* tensor_grid,
                                             std::vector<std::complex<double>>* amplitudes) {
@@ -216,74 +312,10 @@ ContractionData ContractionData::Initialize(const std::list<ContractionOperation
   data.tensor_grid_ = tensor_grid;
   data.amplitudes_ = amplitudes;
 
-  std::unordered_map<std::string, std::size_t> index_dimension;
-  std::vector<std::vector<std::vector<std::string>>> grid_indices(tensor_grid->size());
-  std::size_t max_grid = 0;
-  for (std::size_t i = 0; i < tensor_grid->size(); ++i) {
-    grid_indices[i].resize((*tensor_grid)[i].size());
-    for (std::size_t j = 0; j < (*tensor_grid)[i].size(); ++j) {
-      const Tensor& t = (*tensor_grid)[i][j];
-      grid_indices[i][j] = t.get_indices();
-      for (const auto& index : t.get_indices())
-        index_dimension[index] = t.get_index_to_dimension().at(index);
-      if (t.data()) max_grid = std::max(max_grid, t.size());
-    }
-  }
-
-  // Symbolic pass.  Bond cuts remove their index; terminal cuts keep the
-  // output index open with one entry per requested value.
-  std::unordered_map<std::string, std::vector<std::string>> patch_indices;
-  std::unordered_map<std::string, std::size_t> side_buffers;
-  auto sym_diff = [](const std::vector<std::string>& a, const std::vector<std::string>& b) {
-    return _vector_subtraction(_vector_union(a, b), _vector_intersection(a, b));
-  };
-  auto size_of = [&](const std::vector<std::string>& idx) {
-    std::size_t s = 1;
-    for (const auto& x : idx) s *= index_dimension.at(x);
-    return s;
-  };
-  for (const auto& op : ordering) {
-    switch (op.op_type) {
-      case ContractionOperation::EXPAND: {
-        auto& pi = patch_indices[op.expand.id];
-        pi = sym_diff(pi, grid_indices.at(op.expand.tensor[0]).at(op.expand.tensor[1]));
-        auto& mx = data.patch_max_size_[op.expand.id];
-        mx = std::max(mx, size_of(pi));
-        break;
-      }
-      case ContractionOperation::CUT: {
-        try {
-          const std::string cut_index = index_name(op.cut.tensors);
-          int side = 0;
-          for (const auto& tensor : op.cut.tensors) {
-            auto& indices = grid_indices.at(tensor[0]).at(tensor[1]);
-            const std::size_t full = (*tensor_grid)[tensor[0]][tensor[1]].size();
-            const std::size_t dim = index_dimension.count(cut_index) ? index_dimension[cut_index] : 1;
-            if (op.cut.tensors.size() > 1) {
-              indices = _vector_subtraction(indices, {cut_index});
-              side_buffers[cut_copy_name(op.cut.tensors, side)] = full / std::max<std::size_t>(dim, 1);
-            } else {
-              const std::size_t count = op.cut.values.empty() ? dim : op.cut.values.size();
-              index_dimension[cut_index] = count;
-              side_buffers[cut_copy_name(op.cut.tensors, side)] =
-                  full / std::max<std::size_t>(dim, 1) * count;
-            }
-            ++side;
-          }
-        } catch (const std::string& err_msg) {
-          throw ERROR_MSG("Failed during CUT. Error:\n\t[", err_msg, "]");
-        }
-        break;
-      }
-      case ContractionOperation::MERGE: {
-        auto& pi = patch_indices[op.merge.target_id];
-        pi = sym_diff(pi, patch_indices[op.merge.source_id]);
-        auto& mx = data.patch_max_size_[op.merge.target_id];
-        mx = std::max(mx, size_of(pi));
-        break;
-      }
-    }
-  }
+  SymbolicPlan plan = plan_contraction(ordering, *tensor_grid);
+  data.patch_max_size_ = plan.patch_max_size;
+  const std::size_t max_grid = plan.max_grid;
+  const auto& side_buffers = plan.side_buffers;
 
   std::size_t max_size = max_grid;
   std::vector<std::size_t> unique_sizes;
@@ -679,6 +711,10 @@ void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
     ContractionStats fresh;
     fresh.planned_bytes = stats_.planned_bytes;
     stats_ = fresh;
+    // wall time of the contraction itself: earlier device work (network
+    // build, flatten) is finished first
+    qfx_check(qfx_synchronize(device()), "qfx_synchronize");
+    start_ = std::chrono::steady_clock::now();
     Tensor& amps = get_scratch(kAmplitudeSpace);
     qfx_check(qfx_zero(device(), amps.data(), amps.capacity() * sizeof(s_type)), "qfx_zero");
   }
@@ -697,6 +733,9 @@ void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
                              device_amps.size() * sizeof(std::complex<double>)),
                 "qfx_copy_d2h");
     for (std::size_t i = 0; i < device_amps.size(); ++i) (*amplitudes_)[i] += device_amps[i];
+    qfx_check(qfx_synchronize(device()), "qfx_synchronize");
+    stats_.seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - start_).count();
     g_last_stats = stats_;
   }
 }

@@ -17,6 +17,7 @@
 #ifndef QFLEX_AMD_CONTRACTION_UTILS_H_
 #define QFLEX_AMD_CONTRACTION_UTILS_H_
 
+#include <chrono>
 #include <complex>
 #include <list>
 #include <optional>
@@ -88,7 +89,30 @@ struct ContractionStats {
   std::size_t reorders = 0;
   std::size_t mapped_multiplies = 0;  // contractions read without reorder
   std::size_t planned_bytes = 0;
+  double seconds = 0;  // wall time of the contraction (device work finished)
 };
+
+// One pairwise contraction of an ordering, from the symbolic pass that
+// also plans device memory: C[m x n] = A[m x k] . B[k x n], executed
+// `multiplicity` times (the product of the value counts of the bond cuts
+// before it); out_rank / out_size describe C.
+struct ContractionStep {
+  int op = 0;  // ContractionOperation::EXPAND or MERGE
+  std::size_t m = 1, n = 1, k = 1;
+  std::size_t multiplicity = 1;
+  std::size_t out_rank = 0, out_size = 1;
+};
+
+// The symbolic pass over an ordering (index sets only, no data): patch
+// sizes, cut buffer sizes and the pairwise-contraction steps.
+struct SymbolicPlan {
+  std::unordered_map<std::string, std::size_t> patch_max_size;
+  std::unordered_map<std::string, std::size_t> side_buffers;
+  std::size_t max_grid = 0;
+  std::vector<ContractionStep> steps;
+};
+SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
+                              const std::vector<std::vector<Tensor>>& tensor_grid);
 
 class ContractionData {
  public:
@@ -135,6 +159,7 @@ class ContractionData {
  private:
   std::vector<Tensor> scratch_;
   std::unordered_map<std::string, std::size_t> scratch_map_;
+  std::chrono::steady_clock::time_point start_;
   std::unordered_map<std::string, std::size_t> patch_max_size_;
   std::vector<std::vector<Tensor>>* tensor_grid_ = nullptr;
   std::vector<std::complex<double>>* amplitudes_ = nullptr;

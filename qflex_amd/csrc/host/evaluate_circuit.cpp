@@ -56,8 +56,13 @@ std::string get_output_states(const QflexInput* input,
   return base_state;
 }
 
-std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexInput* input) {
-  if (input == nullptr) throw ERROR_MSG("Input must be non-null.");
+namespace {
+
+// Ordering, output strings and the flattened 2D grid of an input (the part
+// of EvaluateCircuit before ContractGrid).
+void build_network(QflexInput* input, std::list<ContractionOperation>* ordering,
+                   std::vector<std::string>* output_states,
+                   std::vector<std::vector<Tensor>>* tensor_grid_out) {
   using clock = std::chrono::steady_clock;
   auto t0 = clock::now();
   auto lap = [&t0](const char* what) {
@@ -68,23 +73,21 @@ std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexI
       t0 = t1;
     }
   };
-
-  std::list<ContractionOperation> ordering;
-  ordering_data_to_contraction_ordering(*input, &ordering);
+  auto& tensor_grid = *tensor_grid_out;
+  ordering_data_to_contraction_ordering(*input, ordering);
   lap("Time spent making contraction ordering: ");
 
   const std::size_t init_length = input->grid.I * input->grid.J - input->grid.qubits_off.size();
   if (input->initial_state.empty()) input->initial_state = std::string(init_length, '0');
 
   std::vector<std::vector<std::size_t>> final_qubits;
-  std::vector<std::string> output_states;
   try {
-    input->final_state = get_output_states(input, ordering, &final_qubits, &output_states);
+    input->final_state = get_output_states(input, *ordering, &final_qubits, output_states);
   } catch (const std::string& err_msg) {
     throw ERROR_MSG("Failed to call get_output_states(). Error:\n\t[", err_msg, "]");
   }
 
-  std::vector<std::vector<Tensor>> tensor_grid(input->grid.I, std::vector<Tensor>(input->grid.J));
+  tensor_grid.assign(input->grid.I, std::vector<Tensor>(input->grid.J));
   {
     std::vector<std::vector<std::vector<Tensor>>> tensor_grid_3D;
     // Gate tensors have at most 16 elements.
@@ -123,7 +126,7 @@ std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexI
     s_type* scratch_2D = device_alloc(max_size * 2);
     try {
       flatten_grid_of_tensors(tensor_grid_3D, tensor_grid, final_qubits, input->grid.qubits_off,
-                              ordering, scratch_2D);
+                              *ordering, scratch_2D);
     } catch (...) {
       device_free(scratch_2D);
       throw;
@@ -131,6 +134,28 @@ std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexI
     device_free(scratch_2D);
     lap("Time spent creating 2D grid of tensors from 3D one: ");
   }
+
+}
+
+}  // namespace
+
+std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexInput* input) {
+  if (input == nullptr) throw ERROR_MSG("Input must be non-null.");
+  using clock = std::chrono::steady_clock;
+  auto t0 = clock::now();
+  auto lap = [&t0](const char* what) {
+    if (global::verbose > 0) {
+      auto t1 = clock::now();
+      std::cerr << WARN_MSG(what, std::chrono::duration<double>(t1 - t0).count(), "s")
+                << std::endl;
+      t0 = t1;
+    }
+  };
+  std::list<ContractionOperation> ordering;
+  std::vector<std::string> output_states;
+  std::vector<std::vector<Tensor>> tensor_grid;
+  build_network(input, &ordering, &output_states, &tensor_grid);
+  t0 = clock::now();
 
   std::vector<std::complex<double>> amplitudes(output_states.size());
   try {
@@ -143,6 +168,15 @@ std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexI
   for (std::size_t c = 0; c < amplitudes.size(); ++c)
     result.push_back({output_states[c], amplitudes[c]});
   return result;
+}
+
+SymbolicPlan PlanCircuit(QflexInput* input) {
+  if (input == nullptr) throw ERROR_MSG("Input must be non-null.");
+  std::list<ContractionOperation> ordering;
+  std::vector<std::string> output_states;
+  std::vector<std::vector<Tensor>> tensor_grid;
+  build_network(input, &ordering, &output_states, &tensor_grid);
+  return plan_contraction(ordering, tensor_grid);
 }
 
 }  // namespace qflex

@@ -29,6 +29,11 @@ std::string get_output_states(const QflexInput* input,
 // contracts it and returns (bitstring, amplitude) pairs.
 std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexInput* input);
 
+// The same network build, then only the symbolic pass of the contraction
+// (pairwise-contraction shapes and multiplicities, patch sizes): the
+// features of the learned cost model.
+SymbolicPlan PlanCircuit(QflexInput* input);
+
 }  // namespace qflex
 
 #endif

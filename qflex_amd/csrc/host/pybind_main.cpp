@@ -92,8 +92,8 @@ int parse_precision(const std::string& p) {
                   "' (expected fp32, split_fp16, split_bf16x3, split_bf16 or fp32_exact).");
 }
 
-std::vector<std::pair<std::string, std::complex<double>>> evaluate(const py::dict& options) {
-  qflex::QflexInput input;
+// Fills a QflexInput (and the process-wide settings) from a simulate() dict.
+void load_input(const py::dict& options, qflex::QflexInput& input) {
   if (options.contains("memory_limit")) {
     const auto& ml = options["memory_limit"];
     if (py::isinstance<py::int_>(ml)) {
@@ -123,7 +123,39 @@ std::vector<std::pair<std::string, std::complex<double>>> evaluate(const py::dic
     throw ERROR_MSG("Not yet supported");
   input.initial_state = initial_states[0];
   input.final_state = final_states[0];
+}
+
+std::vector<std::pair<std::string, std::complex<double>>> evaluate(const py::dict& options) {
+  qflex::QflexInput input;
+  load_input(options, input);
   return qflex::EvaluateCircuit(&input);
+}
+
+// Symbolic plan of a simulate() dict: one dict per pairwise contraction
+// (op, m, n, k, multiplicity, out_rank, out_size) plus the patch sizes.
+py::dict plan(const py::dict& options) {
+  qflex::QflexInput input;
+  load_input(options, input);
+  const qflex::SymbolicPlan p = qflex::PlanCircuit(&input);
+  py::list steps;
+  for (const auto& st : p.steps) {
+    py::dict d;
+    d["op"] = st.op == qflex::ContractionOperation::EXPAND ? "expand" : "merge";
+    d["m"] = st.m;
+    d["n"] = st.n;
+    d["k"] = st.k;
+    d["multiplicity"] = st.multiplicity;
+    d["out_rank"] = st.out_rank;
+    d["out_size"] = st.out_size;
+    steps.append(d);
+  }
+  py::dict patches;
+  for (const auto& kv : p.patch_max_size) patches[py::str(kv.first)] = kv.second;
+  py::dict out;
+  out["steps"] = steps;
+  out["patch_max_size"] = patches;
+  out["max_grid"] = p.max_grid;
+  return out;
 }
 
 std::vector<std::pair<std::string, std::complex<double>>> simulate(const py::dict& options) {
@@ -211,6 +243,8 @@ py::list from_ops(const std::list<qflex::ContractionOperation>& ops) {
 PYBIND11_MODULE(qflex, m) {
   m.doc() = "MI355X contraction engine for qFlex grid circuits";
   m.def("simulate", &simulate, "Call the simulator with the normal parameters");
+  m.def("plan", [](const py::dict& o) { return wrap([&] { return plan(o); }); },
+        "Symbolic plan (pairwise-contraction shapes) of a simulate() input");
   m.def("evaluate", [](const py::dict& o) { return wrap([&] { return evaluate(o); }); },
         "simulate() that raises RuntimeError instead of printing errors");
 
@@ -407,6 +441,7 @@ PYBIND11_MODULE(qflex, m) {
     d["reorders"] = s.reorders;
     d["reorder_elements"] = s.reorder_elements;
     d["mapped_multiplies"] = s.mapped_multiplies;
+    d["seconds"] = s.seconds;
     d["planned_bytes"] = s.planned_bytes;
     return d;
   });

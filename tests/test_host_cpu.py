@@ -5,6 +5,7 @@ evaluate_circuit}_test.cpp cases)."""
 import json
 import os
 
+import numpy as np
 import pytest
 
 from qflex_amd import qflex
@@ -143,3 +144,40 @@ def test_simulate_swallows_errors_and_returns_empty(capfd):
 def test_simulate_rejects_bad_memory_limit(capfd):
     assert qflex.simulate({"memory_limit": -5}) == []
     assert "'memory_limit' must be positive." in capfd.readouterr().err
+
+
+# ---------------------------------------------------------------- cost-model candidates
+
+@pytest.mark.parametrize("I,J", [(4, 4), (5, 6), (6, 6), (7, 7)])
+def test_costmodel_candidates_are_valid_orderings(I, J):
+    # every candidate (and the plain fallback) parses and passes the
+    # engine's ValidateOrdering; every site is expanded exactly once
+    from qflex_amd import circuits, costmodel
+    rng = np.random.default_rng(I * 10 + J)
+    grid = circuits.grid_text(I, J).splitlines()
+    cands = costmodel.candidate_set(I, J, rng, 40)
+    assert cands[0] == costmodel.plain_ordering(I, J) and len(set(cands)) == len(cands)
+    for order in cands:
+        lines = order.splitlines()
+        ops = qflex.parse_ordering(lines, grid)
+        qflex.validate_ordering(ops)
+        expanded = sorted(int(l.split()[2]) for l in lines if l.startswith("expand"))
+        assert expanded == list(range(I * J))
+        cuts = [l for l in lines if l.startswith("cut")]
+        first_expand = next(n for n, l in enumerate(lines) if l.startswith("expand"))
+        assert all(lines.index(c) < first_expand for c in cuts)
+
+
+def test_costmodel_features_from_a_plan():
+    from qflex_amd import costmodel
+    plan = {"steps": [{"m": 256, "n": 16, "k": 16, "multiplicity": 1, "out_rank": 3,
+                       "out_size": 4096},
+                      {"m": 4096, "n": 256, "k": 256, "multiplicity": 16, "out_rank": 5,
+                       "out_size": 1 << 20}],
+            "patch_max_size": {"a": 1 << 20}, "max_grid": 65536}
+    f = costmodel.features(plan, 6, 6, 24)
+    assert f.shape == (64,) and f.dtype == np.float32 and np.all(np.isfinite(f))
+    total = 8 * (256 * 16 * 16 + 4096 * 256 * 256 * 16)
+    assert abs(f[0] - np.log2(total)) < 1e-5
+    assert f[2] == 20 and f[3] == 5 and f[6] == 4
+    assert abs(f[8:16].sum() - 1) < 1e-6 and abs(f[32:40].sum() - 1) < 1e-6

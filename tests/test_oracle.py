@@ -225,3 +225,54 @@ def test_memory_limit_message():
     with pytest.raises(OracleError, match=r"Required space \(.*\) exceeds memory limit \(1 kB\)"):
         evaluate_circuit(circ, circuits.snake_ordering(3, 3), circuits.grid_text(3, 3),
                          "0" * 9, "0" * 9, memory_limit=1024)
+
+
+# ---------------------------------------------------------------- cost-model network
+
+def test_bf16_rounding_ties_to_even():
+    from oracle.mlp import bf16
+    x = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.5, 1e-3], dtype=np.float32)
+    got = bf16(x)
+    assert got[0] == 1.0 and got[1] == 1.0 and got[2] == 1.0 + 4 * 2 ** -8
+    assert got[3] == -2.5 and abs(got[4] - 1e-3) <= 2 ** -9 * 1e-3
+
+
+def _mlp_case(seed, B=8):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    p = mlp.init_params(seed)
+    p[mlp.N_IN * mlp.N_HID:mlp.N_IN * mlp.N_HID + mlp.N_HID] = rng.standard_normal(mlp.N_HID) * 0.1
+    X = rng.standard_normal((B, mlp.N_IN)).astype(np.float32)
+    t = rng.standard_normal(B).astype(np.float32)
+    return p, X, t
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp
+    p, X, t = _mlp_case(1)
+    loss, g = mlp.loss_and_grads(p, X, t, rounding=False)
+    y, _ = mlp.forward_f64(p.astype(np.float64), X)
+    assert abs(loss - float(np.mean((y - t) ** 2))) < 1e-12
+    rng = np.random.default_rng(2)
+    offsets = np.cumsum([0] + [int(np.prod(s)) for _, s in mlp.SIZES])
+    idx = [int(rng.integers(offsets[b], offsets[b + 1])) for b in range(len(mlp.SIZES))
+           for _ in range(6)]
+    for i in idx:
+        fd = mlp.finite_difference(p, X, t, i)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])), (i, fd, g[i])
+
+
+def test_mlp_finite_differences_catch_a_missing_relu_mask():
+    # planted mistake: layer-1 gradient without the ReLU mask
+    from oracle import mlp
+    p, X, t = _mlp_case(3)
+    P = mlp.unpack(p)
+    y, (Xb, H1, H2) = mlp.forward_f64(p.astype(np.float64), X)
+    dy = 2.0 * (y - t) / len(t)
+    dZ2 = (H2 > 0) * dy[:, None] * P["w3"][None, :]
+    dW1_wrong = Xb.T @ (dZ2 @ P["W2"].astype(np.float64).T)
+    worst = 0.0
+    for i in range(0, mlp.N_IN * mlp.N_HID, 997):
+        fd = mlp.finite_difference(p, X, t, i)
+        worst = max(worst, abs(fd - dW1_wrong.reshape(-1)[i]))
+    assert worst > 1e-4
