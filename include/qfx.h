@@ -143,6 +143,30 @@ int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c64* A,
                          const qfx_c64* B, const qfx_index_map* b_rows,
                          const qfx_index_map* b_cols, qfx_c64* C);
 
+/* Operand scales of QFX_PREC_SPLIT_FP16 across a chain of contractions (no
+ * counterpart in the reference, whose FP32 GEMM needs none).  Each large-tile
+ * GEMM can write one partial maximum of max(|re|,|im|) per block of its
+ * output C into c_max (capacity c_max_cap floats; c_max_n receives the count,
+ * 0 when this call wrote none).  A later GEMM that reads that tensor whole as
+ * A or B passes those partials as a_max/a_max_n (b_max/b_max_n) instead of
+ * having them recomputed by a pass over the operand.  NULL members are
+ * ignored. */
+typedef struct qfx_scale_io {
+  const float* a_max;
+  int a_max_n;
+  const float* b_max;
+  int b_max_n;
+  float* c_max;
+  int c_max_cap;
+  int c_max_n; /* out */
+} qfx_scale_io;
+
+/* qfx_cgemm_mapped_c64 with the scale chain above (io may be NULL). */
+int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qfx_c64* A,
+                            const qfx_index_map* a_rows, const qfx_index_map* a_cols,
+                            const qfx_c64* B, const qfx_index_map* b_rows,
+                            const qfx_index_map* b_cols, qfx_c64* C, qfx_scale_io* io);
+
 /* trans == 0: y[m] = A[m x k] . x[k]  (cblas_cgemv NoTrans in _multiply_Mv,
  *             src/tensor.cpp:982-1002)
  * trans == 1: y[k] = sum_r x[r] A[r][.], A is [m x k], x has m entries

@@ -28,6 +28,14 @@ class IndexMap(ctypes.Structure):
                 ("shift", ctypes.c_ubyte * 16)]
 
 
+class ScaleIO(ctypes.Structure):
+    """qfx_scale_io: split-FP16 partial maxima passed along a chain of GEMMs."""
+    _fields_ = [("a_max", ctypes.c_void_p), ("a_max_n", ctypes.c_int),
+                ("b_max", ctypes.c_void_p), ("b_max_n", ctypes.c_int),
+                ("c_max", ctypes.c_void_p), ("c_max_cap", ctypes.c_int),
+                ("c_max_n", ctypes.c_int)]
+
+
 def index_map(dims, axes):
     """Map of the matrix index over tensor axes `axes` (most significant
     first) of a row-major tensor with power-of-two `dims`."""
@@ -64,6 +72,9 @@ SIGNATURES = {
     "qfx_cgemm_mapped_c64": (_c.c_int, [_vp, _c.c_int, _vp, _c.POINTER(IndexMap),
                                         _c.POINTER(IndexMap), _vp, _c.POINTER(IndexMap),
                                         _c.POINTER(IndexMap), _vp]),
+    "qfx_cgemm_mapped_ex_c64": (_c.c_int, [_vp, _c.c_int, _vp, _c.POINTER(IndexMap),
+                                           _c.POINTER(IndexMap), _vp, _c.POINTER(IndexMap),
+                                           _c.POINTER(IndexMap), _vp, _c.POINTER(ScaleIO)]),
     "qfx_cgemv_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _vp, _vp, _vp]),
     "qfx_cdotu_c64": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
     "qfx_accumulate_c128": (_c.c_int, [_vp, _vp, _u64, _vp, _u64]),
@@ -183,6 +194,15 @@ class Context:
     def mlp_adam(self, params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         check(load().qfx_mlp_adam(self.handle, params.ptr, grads.ptr, m.ptr, v.ptr, step, lr,
                                   beta1, beta2, eps))
+
+    def cgemm_mapped_ex(self, precision, A, a_rows, a_cols, B, b_rows, b_cols, C, io):
+        st = load().qfx_cgemm_mapped_ex_c64(self.handle, precision, A.ptr, ctypes.byref(a_rows),
+                                            ctypes.byref(a_cols), B.ptr, ctypes.byref(b_rows),
+                                            ctypes.byref(b_cols), C.ptr, ctypes.byref(io))
+        if st == 4:
+            return False
+        check(st)
+        return True
 
     def cgemv(self, trans, m, k, A, x, y):
         check(load().qfx_cgemv_c64(self.handle, trans, m, k, A.ptr, x.ptr, y.ptr))

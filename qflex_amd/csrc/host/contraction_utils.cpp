@@ -362,6 +362,18 @@ ContractionData ContractionData::Initialize(const std::list<ContractionOperation
   for (const auto& p : data.patch_max_size_) add(p.first, p.second);
   for (const auto& p : side_buffers) add(p.first, p.second);
   add(kAmplitudeSpace, 2 * amplitudes->size());
+  // Partial-maximum arrays beside every buffer a GEMM writes a patch into
+  // (one float per output block; a block holds at least 2^14 outputs).
+  std::vector<std::string> gemm_outputs;
+  for (auto s : unique_sizes) gemm_outputs.push_back(result_space(s));
+  for (const auto& p : data.patch_max_size_) gemm_outputs.push_back(p.first);
+  data.amax_store_.reserve(gemm_outputs.size());
+  for (const auto& name : gemm_outputs) {
+    const Tensor& buf = data.scratch_[data.scratch_map_.at(name)];
+    const int cap = int(buf.capacity() / 8192 + 64);
+    data.amax_store_.push_back(Tensor({""}, {std::size_t(cap + 1) / 2}));
+    data.amax_slots_[buf.data()] = {reinterpret_cast<float*>(data.amax_store_.back().data()), cap};
+  }
   return data;
 }
 
@@ -405,17 +417,18 @@ bool ContractionData::try_mapped(const s_type* a, const std::vector<std::string>
                                  const std::vector<std::size_t>& b_dims,
                                  const std::vector<std::string>& left,
                                  const std::vector<std::string>& common,
-                                 const std::vector<std::string>& right, s_type* out) {
+                                 const std::vector<std::string>& right, s_type* out,
+                                 qfx_scale_io* io) {
   qfx_index_map ar, ac, br, bc;
   if (!build_index_map(a_indices, a_dims, left, &ar) ||
       !build_index_map(a_indices, a_dims, common, &ac) ||
       !build_index_map(b_indices, b_dims, common, &br) ||
       !build_index_map(b_indices, b_dims, right, &bc))
     return false;
-  const int st = qfx_cgemm_mapped_c64(device(), global::precision,
-                                      reinterpret_cast<const qfx_c64*>(a), &ar, &ac,
-                                      reinterpret_cast<const qfx_c64*>(b), &br, &bc,
-                                      reinterpret_cast<qfx_c64*>(out));
+  const int st = qfx_cgemm_mapped_ex_c64(device(), global::precision,
+                                         reinterpret_cast<const qfx_c64*>(a), &ar, &ac,
+                                         reinterpret_cast<const qfx_c64*>(b), &br, &bc,
+                                         reinterpret_cast<qfx_c64*>(out), io);
   if (st == QFX_ERR_UNSUPPORTED) return false;
   qfx_check(st, "qfx_cgemm_mapped_c64");
   if (global::verbose > 1) {
@@ -452,11 +465,23 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
   std::vector<std::string> b_indices = B.indices;
   std::vector<std::size_t> b_dims = B.dims;
   bool done = false;
-  if ((a_order != A.indices || b_order != B.indices) && !left.empty() && !right.empty() &&
-      global::mapped_operands) {
+  // split-FP16 scales: the operands' partial maxima if their producing GEMM
+  // wrote them, and the output's if `out` has an array for them
+  qfx_scale_io io{};
+  io.a_max = A.amax;
+  io.a_max_n = A.amax_n;
+  io.b_max = B.amax;
+  io.b_max_n = B.amax_n;
+  auto slot = amax_slots_.find(out);
+  if (slot != amax_slots_.end()) {
+    io.c_max = slot->second.first;
+    io.c_max_cap = slot->second.second;
+  }
+  if (!left.empty() && !right.empty() && global::mapped_operands) {
     // Read A (and B) in stored order; if B's layout is the obstacle and B is
     // the smaller operand, reorder only B and try again.
-    done = try_mapped(A.ptr, A.indices, A.dims, pb, b_indices, b_dims, left, common, right, out);
+    done = try_mapped(A.ptr, A.indices, A.dims, pb, b_indices, b_dims, left, common, right, out,
+                      &io);
     if (!done && b_order != B.indices && B.size() <= A.size()) {
       s_type* buf = get_scratch(kGeneralSpaceB).data();
       device_reorder(B.ptr, buf, B.indices, B.dims, b_order);
@@ -466,9 +491,11 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
       for (const auto& x : b_order) b_dims.push_back(bdim[x]);
       stats_.reorders++;
       stats_.reorder_elements += double(B.size());
-      done = try_mapped(A.ptr, A.indices, A.dims, pb, b_indices, b_dims, left, common, right, out);
+      done = try_mapped(A.ptr, A.indices, A.dims, pb, b_indices, b_dims, left, common, right, out,
+                        &io);
     }
   }
+  if (!done) io.c_max_n = 0;
   if (!done) {
     if (a_order != A.indices) {
       s_type* buf = get_scratch(kGeneralSpace).data();
@@ -494,6 +521,8 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
   stats_.flops += 8.0 * double(left_dim) * double(right_dim) * double(common_dim);
   View C;
   C.ptr = out;
+  C.amax = io.c_max_n > 0 ? io.c_max : nullptr;
+  C.amax_n = io.c_max_n;
   C.indices = _vector_union(left, right);
   for (const auto& x : left) C.dims.push_back(adim[x]);
   for (const auto& x : right) C.dims.push_back(bdim[x]);

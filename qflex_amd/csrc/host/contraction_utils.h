@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "global.h"
+#include "qfx.h"
 #include "input.h"
 #include "tensor.h"
 #include "utils.h"
@@ -149,6 +150,10 @@ class ContractionData {
     const s_type* ptr = nullptr;
     std::vector<std::string> indices;
     std::vector<std::size_t> dims;
+    // partial maxima of |re|,|im| written by the GEMM that produced this
+    // tensor (split-FP16 scales), or none
+    const float* amax = nullptr;
+    int amax_n = 0;
     std::size_t size() const;
   };
   struct OpenOutput {
@@ -158,6 +163,10 @@ class ContractionData {
 
  private:
   std::vector<Tensor> scratch_;
+  // per result/patch buffer: device array for the partial maxima of the
+  // tensor a GEMM writes there (pointer, capacity in floats)
+  std::vector<Tensor> amax_store_;
+  std::unordered_map<const s_type*, std::pair<float*, int>> amax_slots_;
   std::unordered_map<std::string, std::size_t> scratch_map_;
   std::chrono::steady_clock::time_point start_;
   std::unordered_map<std::string, std::size_t> patch_max_size_;
@@ -176,7 +185,7 @@ class ContractionData {
                   const std::vector<std::size_t>& a_dims, const s_type* b,
                   const std::vector<std::string>& b_indices, const std::vector<std::size_t>& b_dims,
                   const std::vector<std::string>& left, const std::vector<std::string>& common,
-                  const std::vector<std::string>& right, s_type* out);
+                  const std::vector<std::string>& right, s_type* out, qfx_scale_io* io);
   View contract_pair(const View& A, const View& B, s_type* out);
   View project_view(const View& v, const std::string& index, std::size_t value,
                     const std::vector<std::size_t>* gather_values, s_type* gather_buffer);

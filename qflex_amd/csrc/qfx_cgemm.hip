@@ -367,10 +367,13 @@ __global__ __launch_bounds__(256) void amax_partial_kernel(const float2* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
 }
 
-// Partial maxima of the operands of one split-FP16 GEMM.
+// Partial maxima of the operands of one split-FP16 GEMM (from
+// amax_partial_kernel, or written by the GEMM that produced the operand).
 struct AmaxRef {
-  const float* part;  // na partials of A, then nb of B
-  int na, nb;
+  const float* a;  // na partial maxima of |A|
+  int na;
+  const float* b;  // nb partial maxima of |B|
+  int nb;
 };
 
 // Power of two s with max|x| * s in [2^14, 2^15) (FP16 max is 65504).
@@ -389,8 +392,8 @@ __device__ __forceinline__ void operand_scales(const AmaxRef& r, float* red, flo
                                                float& sb) {
   constexpr int W = THREADS / 64;
   float ma = 0.f, mb = 0.f;
-  for (int i = threadIdx.x; i < r.na; i += THREADS) ma = fmaxf(ma, r.part[i]);
-  for (int i = threadIdx.x; i < r.nb; i += THREADS) mb = fmaxf(mb, r.part[r.na + i]);
+  for (int i = threadIdx.x; i < r.na; i += THREADS) ma = fmaxf(ma, r.a[i]);
+  for (int i = threadIdx.x; i < r.nb; i += THREADS) mb = fmaxf(mb, r.b[i]);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     ma = fmaxf(ma, __shfl_xor(ma, off));
@@ -641,7 +644,8 @@ template <int MODE, int WM, int WN, int J, int AM>
 __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
     int N, int K, AmaxRef amax, qfx_index_map a_rows,
-    qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols) {
+    qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols,
+    float* __restrict__ cmax) {
   static_assert(WM * WN == 8, "8 waves");
   typedef typename SplitTraits<MODE>::T T;
   typedef typename SplitTraits<MODE>::V V;
@@ -892,6 +896,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
 
   const float unscale = 1.f / (sa * sb);
   float* Cf = reinterpret_cast<float*>(C);
+  float cm = 0.f;  // max |re|, |im| of the values this thread stores
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -900,8 +905,27 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
       for (int r = 0; r < 16; ++r) {
         const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
         const int cn = n0 + wn * 16 * J + j * 16 + (col >> 1);
-        if (row < M && cn < N) Cf[(uint64_t(row) * N + cn) * 2 + part] = acc[i][j][r] * unscale;
+        if (row < M && cn < N) {
+          const float v = acc[i][j][r] * unscale;
+          Cf[(uint64_t(row) * N + cn) * 2 + part] = v;
+          cm = fmaxf(cm, fabsf(v));
+        }
       }
+  if (cmax) {
+    // one partial maximum per block, for the split of the GEMM that reads C
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cm = fmaxf(cm, __shfl_xor(cm, off));
+    float* red = reinterpret_cast<float*>(smem_raw);
+    __syncthreads();  // the last stage's fragment reads are done
+    if (lane == 0) red[wave] = cm;
+    __syncthreads();
+    if (tid == 0) {
+      float b = red[0];
+#pragma unroll
+      for (int w = 1; w < 8; ++w) b = fmaxf(b, red[w]);
+      cmax[blockIdx.x + gridDim.x * blockIdx.y] = b;
+    }
+  }
 }
 
 template <int MODE, int WM, int WN, int J>
@@ -931,7 +955,7 @@ hipError_t split_big_prepare() {
 template <int MODE, int WM, int WN, int J, int AM>
 hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
                             uint64_t m, uint64_t n, uint64_t k, AmaxRef amax,
-                            const qfx_index_map* maps) {
+                            const qfx_index_map* maps, float* cmax) {
   hipError_t e = split_big_prepare<MODE, WM, WN, J, AM>();
   if (e != hipSuccess) return e;
   constexpr int kBM = 64 * WM, kBN = 16 * J * WN;
@@ -940,7 +964,7 @@ hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B
   cgemm_split_big_kernel<MODE, WM, WN, J, AM>
       <<<g, 512, split_big_lds<MODE, WM, WN, J, AM>(k), stream>>>(
           A, B, C, int(m), int(n), int(k), amax, maps ? maps[0] : none, maps ? maps[1] : none,
-          maps ? maps[2] : none, maps ? maps[3] : none);
+          maps ? maps[2] : none, maps ? maps[3] : none, cmax);
   return hipGetLastError();
 }
 
@@ -973,15 +997,15 @@ template <int MODE, int AM>
 hipError_t launch_split_big_layout(BigLayout lay, hipStream_t stream, const float2* A,
                                    const float2* B, float2* C, uint64_t m, uint64_t n,
                                    uint64_t k, AmaxRef amax,
-                                   const qfx_index_map* maps) {
+                                   const qfx_index_map* maps, float* cmax) {
   if constexpr (MODE == 0) {
-    return launch_split_big<0, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
+    return launch_split_big<0, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
   } else {
     if (lay == kLayout2x4)
-      return launch_split_big<MODE, 2, 4, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
+      return launch_split_big<MODE, 2, 4, 4, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
     if (lay == kLayout8x1)
-      return launch_split_big<MODE, 8, 1, 2, AM>(stream, A, B, C, m, n, k, amax, maps);
-    return launch_split_big<MODE, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps);
+      return launch_split_big<MODE, 8, 1, 2, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
+    return launch_split_big<MODE, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
   }
 }
 
@@ -997,12 +1021,19 @@ template <int AM>
 hipError_t launch_split_big_prec(int precision, BigLayout lay, hipStream_t stream,
                                  const float2* A, const float2* B, float2* C, uint64_t m,
                                  uint64_t n, uint64_t k, AmaxRef amax,
-                                 const qfx_index_map* maps) {
+                                 const qfx_index_map* maps, float* cmax) {
   if (precision == QFX_PREC_SPLIT_BF16X3)
-    return launch_split_big_layout<0, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
+    return launch_split_big_layout<0, AM>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
   if (precision == QFX_PREC_SPLIT_BF16)
-    return launch_split_big_layout<1, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
-  return launch_split_big_layout<2, AM>(lay, stream, A, B, C, m, n, k, amax, maps);
+    return launch_split_big_layout<1, AM>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
+  return launch_split_big_layout<2, AM>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
+}
+
+// Blocks (one partial maximum each) of a large-tile launch.
+uint64_t big_layout_blocks(BigLayout lay, uint64_t m, uint64_t n) {
+  const uint64_t bm = lay == kLayout2x4 ? 128 : (lay == kLayout8x1 ? 512 : 256);
+  const uint64_t bn = lay == kLayout2x4 ? 256 : (lay == kLayout8x1 ? 32 : 128);
+  return ((m + bm - 1) / bm) * ((n + bn - 1) / bn);
 }
 
 // ------------------------------------------------------------ GEMV / dot
@@ -1500,17 +1531,23 @@ extern "C" int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, con
 
 // Operand scales of the split-FP16 form: partial maxima of |A| and |B| in
 // one launch (at most 256 blocks per operand, at least 16 Ki elements
-// each); the GEMM kernel reduces them (no host round trip, no atomics).
+// each), skipping an operand whose maxima its producer already wrote; the
+// GEMM kernel reduces them (no host round trip, no atomics).
 static int launch_amax(qfx_context* ctx, const float2* A, uint64_t na, const float2* B,
-                       uint64_t nb, AmaxRef* ref) {
-  const int ba = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (na + 16383) / 16384)));
-  const int bb = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (nb + 16383) / 16384)));
-  void* ws = nullptr;
-  int st = qfx_workspace(ctx, size_t(ba + bb) * sizeof(float), &ws);
-  if (st != QFX_OK) return st;
-  float* part = static_cast<float*>(ws);
-  amax_partial_kernel<<<unsigned(ba + bb), 256, 0, ctx->stream>>>(A, na, B, nb, ba, part);
-  *ref = AmaxRef{part, ba, bb};
+                       uint64_t nb, AmaxRef* ref, const float* a_known = nullptr, int a_n = 0,
+                       const float* b_known = nullptr, int b_n = 0) {
+  const int ba = a_known ? 0 : int(std::min<uint64_t>(256, std::max<uint64_t>(1, (na + 16383) / 16384)));
+  const int bb = b_known ? 0 : int(std::min<uint64_t>(256, std::max<uint64_t>(1, (nb + 16383) / 16384)));
+  float* part = nullptr;
+  if (ba + bb > 0) {
+    void* ws = nullptr;
+    int st = qfx_workspace(ctx, size_t(ba + bb) * sizeof(float), &ws);
+    if (st != QFX_OK) return st;
+    part = static_cast<float*>(ws);
+    amax_partial_kernel<<<unsigned(ba + bb), 256, 0, ctx->stream>>>(A, na, B, nb, ba, part);
+  }
+  *ref = AmaxRef{a_known ? a_known : part, a_known ? a_n : ba, b_known ? b_known : part + ba,
+                 b_known ? b_n : bb};
   return QFX_OK;
 }
 
@@ -1611,7 +1648,7 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
        (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0))
           ? choose_big_layout(precision, m, n)
           : kLayoutNone;
-  AmaxRef amax{nullptr, 0, 0};
+  AmaxRef amax{nullptr, 0, nullptr, 0};
   if (precision == QFX_PREC_SPLIT_FP16) {
     int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax);
     if (st != QFX_OK) return st;
@@ -1619,7 +1656,7 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   if (layout != kLayoutNone) {
     if ((n + 31) / 32 > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
     QFX_HIP(launch_split_big_prec<0>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                     nullptr));
+                                     nullptr, nullptr));
     return QFX_OK;
   }
   if (precision == QFX_PREC_SPLIT_FP16) {
@@ -1662,7 +1699,18 @@ extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c
                                     const qfx_index_map* a_rows, const qfx_index_map* a_cols,
                                     const qfx_c64* B, const qfx_index_map* b_rows,
                                     const qfx_index_map* b_cols, qfx_c64* C) {
+  return qfx_cgemm_mapped_ex_c64(ctx, precision, A, a_rows, a_cols, B, b_rows, b_cols, C,
+                                 nullptr);
+}
+
+extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qfx_c64* A,
+                                       const qfx_index_map* a_rows,
+                                       const qfx_index_map* a_cols, const qfx_c64* B,
+                                       const qfx_index_map* b_rows,
+                                       const qfx_index_map* b_cols, qfx_c64* C,
+                                       qfx_scale_io* io) {
   QFX_CHECK_CTX(ctx);
+  if (io) io->c_max_n = 0;
   if (precision < QFX_PREC_FP32 || precision > QFX_PREC_SPLIT_FP16)
     return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: unknown precision");
   uint64_t m, k, k2, n;
@@ -1728,17 +1776,27 @@ extern "C" int qfx_cgemm_mapped_c64(qfx_context* ctx, int precision, const qfx_c
       !split_big_mapped_fits(precision, layout, k))
     return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_mapped_c64: no kernel reads this form");
   QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
-  AmaxRef amax{nullptr, 0, 0};
+  AmaxRef amax{nullptr, 0, nullptr, 0};
+  float* cmax = nullptr;
   if (precision == QFX_PREC_SPLIT_FP16) {
-    int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax);
+    const bool a_known = io && io->a_max && io->a_max_n > 0;
+    const bool b_known = io && io->b_max && io->b_max_n > 0;
+    int st = launch_amax(ctx, A2, m * k, B2, k * n, &amax, a_known ? io->a_max : nullptr,
+                         a_known ? io->a_max_n : 0, b_known ? io->b_max : nullptr,
+                         b_known ? io->b_max_n : 0);
     if (st != QFX_OK) return st;
+    const uint64_t blocks = big_layout_blocks(layout, m, n);
+    if (io && io->c_max && blocks <= uint64_t(std::max(io->c_max_cap, 0))) {
+      cmax = io->c_max;
+      io->c_max_n = int(blocks);
+    }
   }
   const qfx_index_map maps[4] = {*a_rows, *a_cols, *b_rows, *b_cols};
   if (am == 1)
     QFX_HIP(launch_split_big_prec<1>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                     maps));
+                                     maps, cmax));
   else
     QFX_HIP(launch_split_big_prec<2>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                     maps));
+                                     maps, cmax));
   return QFX_OK;
 }

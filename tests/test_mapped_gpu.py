@@ -137,3 +137,32 @@ def test_mapped_unsupported_forms(ctx):
                                                 "right", seed=5))
     assert res[capi.SPLIT_FP16] is None
 
+
+
+def test_scale_chain_matches_recomputed_scales(ctx):
+    # GEMM 1 writes per-block maxima of |C1|; GEMM 2 reads C1 as A with
+    # those maxima instead of a pass over C1: the same scale, so the same bits
+    rng = np.random.default_rng(12)
+    m, k1, k2, n = 65536, 32, 256, 128
+    a = (rng.standard_normal((m, k1)) + 1j * rng.standard_normal((m, k1))).astype(np.complex64)
+    b1 = (rng.standard_normal((k1, k2)) + 1j * rng.standard_normal((k1, k2))).astype(np.complex64)
+    b2 = (rng.standard_normal((k2, n)) + 1j * rng.standard_normal((k2, n))).astype(np.complex64)
+    A, B1, B2 = ctx.upload(a), ctx.upload(b1), ctx.upload(b2)
+    C1, C2, C2ref = ctx.alloc(m * k2 * 8), ctx.alloc(m * n * 8), ctx.alloc(m * n * 8)
+    cap = 4096
+    CM = ctx.alloc(4 * cap)
+    nat = lambda r, c: (capi.index_map([r, c], [0]), capi.index_map([r, c], [1]))
+    io1 = capi.ScaleIO(None, 0, None, 0, CM.ptr, cap, 0)
+    assert ctx.cgemm_mapped_ex(capi.SPLIT_FP16, A, *nat(m, k1), B1, *nat(k1, k2), C1, io1)
+    assert 0 < io1.c_max_n <= cap
+    cmax = ctx.download(CM, np.float32, (io1.c_max_n,))
+    c1 = ctx.download(C1, np.complex64, (m, k2))
+    assert cmax.max() == max(np.abs(c1.real).max(), np.abs(c1.imag).max())
+    io2 = capi.ScaleIO(CM.ptr, io1.c_max_n, None, 0, None, 0, 0)
+    assert ctx.cgemm_mapped_ex(capi.SPLIT_FP16, C1, *nat(m, k2), B2, *nat(k2, n), C2, io2)
+    assert ctx.cgemm_mapped(capi.SPLIT_FP16, C1, *nat(m, k2), B2, *nat(k2, n), C2ref)
+    got = ctx.download(C2, np.complex64, (m, n))
+    ref = ctx.download(C2ref, np.complex64, (m, n))
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    for buf in (A, B1, B2, C1, C2, C2ref, CM):
+        buf.free()
