@@ -8,10 +8,12 @@ contract all 256 cut slices of the corner-inward ordering and deliver the
 
 Precision variants of the complex GEMMs: fp32 (FP32 MFMA), split_fp16
 (operands scaled by powers of two, FP16 hi+lo, 3 products, FP32
-accumulate), split_bf16x3 (BF16 hi+mid+lo, 6 products) and split_bf16
-(hi+mid, 3 products).  Each is timed; its norm-wise relative error is measured on a
-16-slice sub-job against committed complex128 oracle amplitudes.  `value`
-is the fastest variant whose error is <= 1e-5.
+accumulate), split_bf16x3 (BF16 hi+mid+lo, 6 products), split_bf16
+(hi+mid, 3 products) and fp8 (the low-fidelity mode: E4M3 operands with one
+power-of-two scale per 128x128 tile).  Each is timed; its norm-wise relative
+error (and fidelity |<ref|psi>|^2 / (|ref|^2 |psi|^2)) is measured on the
+full job against committed complex128 oracle amplitudes.  `value` is the
+fastest variant whose error is <= 1e-5.
 
 Printed: ONE JSON line (rank 0).  --gpus N > 1 is refused: this build runs
 on one GPU.
@@ -27,7 +29,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "amplitudes/s, 7x7 depth-32 grid: plain FP32 vs split-BF16 at rel.err≤1e-5"
-PEAK_TFLOPS = {"fp32": 157.3, "split_fp16": 2500.0, "split_bf16x3": 2500.0, "split_bf16": 2500.0}
+PEAK_TFLOPS = {"fp32": 157.3, "split_fp16": 2500.0, "split_bf16x3": 2500.0, "split_bf16": 2500.0,
+               "fp8": 5000.0}
 HBM_GBPS = 8000.0
 ERR_BAR = 1e-5
 
@@ -55,13 +58,21 @@ def relerr(got, ref):
     return float(np.linalg.norm(g - r) / np.linalg.norm(r))
 
 
+def fidelity(got, ref):
+    import numpy as np
+    g = np.array(got, dtype=complex)
+    r = np.array(ref, dtype=complex)
+    return float(abs(np.vdot(r, g)) ** 2 / (np.vdot(r, r).real * np.vdot(g, g).real))
+
+
 def measure_error(qflex, circ, grid, precision):
-    """Norm-wise relative error vs the committed complex128 oracle amplitudes
-    of a sub-job (same circuit, ordering restricted to a subset of slices)."""
+    """Norm-wise relative error and fidelity vs the committed complex128
+    oracle amplitudes of the largest committed (sub-)job (same circuit; a
+    sub-job restricts the ordering's cuts to a subset of their values)."""
     files = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "headline_7x7_d32_seed0_slices*.json")),
                    key=lambda p: json.load(open(p))["slices"])
     if not files:
-        return None, 0
+        return None, 0, None
     from qflex_amd import circuits
     ref = json.load(open(files[-1]))
     order = circuits.corner_inward_7x7(tuple(ref["cut_values"]))
@@ -69,7 +80,7 @@ def measure_error(qflex, circ, grid, precision):
     got = {s: a for s, a in r}
     refs = [complex(re, im) for _, re, im in ref["amplitudes"]]
     vals = [got[s] for s, _, _ in ref["amplitudes"]]
-    return relerr(vals, refs), ref["slices"]
+    return relerr(vals, refs), ref["slices"], fidelity(vals, refs)
 
 
 def time_variant(qflex, opts, steps, warmup):
@@ -250,7 +261,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variants", default="fp32,split_fp16,split_bf16x3,split_bf16")
+    ap.add_argument("--variants", default="fp32,split_fp16,split_bf16x3,split_bf16,fp8")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--microbench", default="",
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
@@ -295,7 +306,7 @@ def main():
         log(f"[bench] variant {prec}: warmup {args.warmup}, steps {args.steps}")
         dt, n_amps, prof, stats, dt_prof = time_variant(qflex, options(circ, order, grid, prec),
                                                         args.steps, args.warmup)
-        err, err_slices = measure_error(qflex, circ, grid, prec)
+        err, err_slices, fid = measure_error(qflex, circ, grid, prec)
         g = prof["cgemm"]
         p = prof["permute"]
         variants[prec] = {
@@ -303,6 +314,7 @@ def main():
             "ms_per_step": 1e3 * dt / args.steps,
             "relerr_vs_complex128": err,
             "relerr_slices": err_slices,
+            "fidelity": fid,
             "profiled_ms_per_step": 1e3 * dt_prof,
             # per-class HIP-event totals of the profiled step
             "cgemm": {"launches": g["launches"], "ms": g["ms"],

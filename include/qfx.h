@@ -51,8 +51,13 @@ enum qfx_precision {
   QFX_PREC_SPLIT_BF16 = 1, /* A,B = hi+mid BF16; hi*hi + hi*mid + mid*hi, FP32 acc */
   QFX_PREC_FP32_EXACT = 2, /* scalar fmaf in K order (defined reduction order) */
   QFX_PREC_SPLIT_BF16X3 = 3, /* A,B = hi+mid+lo BF16; 6 cross products, FP32 acc */
-  QFX_PREC_SPLIT_FP16 = 4    /* A,B scaled by powers of two to max|x| in [2^14, 2^15),
+  QFX_PREC_SPLIT_FP16 = 4,   /* A,B scaled by powers of two to max|x| in [2^14, 2^15),
                                 split into FP16 hi + lo; hi*hi + hi*lo + lo*hi, FP32 acc */
+  QFX_PREC_FP8 = 5           /* low fidelity: A and the real 2x2 expansion of B as OCP
+                                E4M3, one power-of-two scale per 128x128 tile (max scaled
+                                value < 256), FP32 acc.  Small products run as in the other
+                                precisions (exact FP32 for k <= 16 or m*n <= 2^16); the
+                                rest below m 128, n 64, k 64 run as QFX_PREC_FP32 */
 };
 
 /* Kernel classes reported by qfx_profile_read. */

@@ -21,7 +21,7 @@ LIBDIR = os.path.join(PKG, "lib")
 BUILD = os.path.join(ROOT, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["qfx_runtime.hip", "qfx_permute.hip", "qfx_cgemm.hip", "qfx_mlp.hip"]
+HIP_SOURCES = ["qfx_runtime.hip", "qfx_permute.hip", "qfx_cgemm.hip", "qfx_mlp.hip", "qfx_fp8.hip"]
 HOST_SOURCES = ["errors.cpp", "tensor.cpp", "contraction_utils.cpp", "read_circuit.cpp",
                 "evaluate_circuit.cpp", "circuit.cpp", "grid.cpp", "ordering.cpp"]
 PYBIND_SOURCE = "pybind_main.cpp"

@@ -10,9 +10,9 @@ import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libqfx.so")
 
-FP32, SPLIT_BF16, FP32_EXACT, SPLIT_BF16X3, SPLIT_FP16 = 0, 1, 2, 3, 4
+FP32, SPLIT_BF16, FP32_EXACT, SPLIT_BF16X3, SPLIT_FP16, FP8 = 0, 1, 2, 3, 4, 5
 PRECISIONS = {"fp32": FP32, "split_bf16": SPLIT_BF16, "fp32_exact": FP32_EXACT,
-              "split_bf16x3": SPLIT_BF16X3, "split_fp16": SPLIT_FP16}
+              "split_bf16x3": SPLIT_BF16X3, "split_fp16": SPLIT_FP16, "fp8": FP8}
 K_PERMUTE, K_CGEMM, K_GEMV, K_DOT, K_ACCUMULATE, K_COPY, K_CGEMM_SMALL = range(7)
 KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy", "cgemm_small"]
 MLP_IN, MLP_HIDDEN, MLP_PARAMS = 64, 256, 82689

@@ -620,19 +620,6 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-// Offset of matrix index idx under a digit map (see qfx_index_map).  The
-// map is a bit permutation, so for i0 a multiple of 2^b and d < 2^b,
-// offset(i0 + d) = offset(i0) + offset(d).
-__device__ __forceinline__ uint64_t map_offset(const qfx_index_map& d, uint64_t idx) {
-  uint64_t off = 0;
-  for (int j = 0; j < d.ndigits; ++j) {
-    const int b = d.bits[j];
-    off += (idx & ((uint64_t(1) << b) - 1)) << d.shift[j];
-    idx >>= b;
-  }
-  return off;
-}
-
 // Operand addressing of the large-tile kernel.
 //   AM 0: A row-major m x k, B row-major k x n.
 //   AM 1: A and B through digit maps; A read along k (a_cols digit 0 at
@@ -1554,7 +1541,7 @@ static int launch_amax(qfx_context* ctx, const float2* A, uint64_t na, const flo
 extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n,
                              uint64_t k, const qfx_c64* A, const qfx_c64* B, qfx_c64* C) {
   QFX_CHECK_CTX(ctx);
-  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_SPLIT_FP16)
+  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_FP8)
     return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: unknown precision");
   if (m == 0 || n == 0) return QFX_OK;
   if (!C) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: null C");
@@ -1601,6 +1588,10 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     QFX_HIP(hipGetLastError());
     return QFX_OK;
   }
+  if (precision == QFX_PREC_FP8 && qfx_fp8_applies(m, n, k)) {
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
+    return qfx_cgemm_fp8(ctx, A2, B2, C2, m, n, k, nullptr);
+  }
   const bool big = m >= 64 && n >= 16 && k >= 8 && m <= 0x7fffffffULL &&
                    n <= 0x7fffffffULL && k <= 0x7fffffffULL;
   if (precision == QFX_PREC_FP32_EXACT || !big) {
@@ -1644,7 +1635,8 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   // Large tiles (one block per CU) when there are enough of them to fill
   // the chip; float4 operand loads need even k and n and 16-byte alignment.
   const BigLayout layout =
-      (precision != QFX_PREC_FP32 && (k % 2 == 0) && (n % 2 == 0) && big_operands_fit(m, n, k) &&
+      (precision != QFX_PREC_FP32 && precision != QFX_PREC_FP8 && (k % 2 == 0) && (n % 2 == 0) &&
+       big_operands_fit(m, n, k) &&
        (reinterpret_cast<uintptr_t>(A) % 16 == 0) && (reinterpret_cast<uintptr_t>(B) % 16 == 0))
           ? choose_big_layout(precision, m, n)
           : kLayoutNone;
@@ -1665,7 +1657,8 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     QFX_HIP(hipGetLastError());
     return QFX_OK;
   }
-  if (precision == QFX_PREC_FP32)
+  // FP8 below its tile sizes runs the FP32 MFMA kernel.
+  if (precision == QFX_PREC_FP32 || precision == QFX_PREC_FP8)
     cgemm_f32_mfma_kernel<<<grid, 256, 0, ctx->stream>>>(A2, B2, C2, int(m), int(n), int(k));
   else if (precision == QFX_PREC_SPLIT_BF16)
     cgemm_bf16_mfma_kernel<3><<<grid, 256, 0, ctx->stream>>>(A2, B2, C2, int(m), int(n), int(k));
@@ -1711,7 +1704,7 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
                                        qfx_scale_io* io) {
   QFX_CHECK_CTX(ctx);
   if (io) io->c_max_n = 0;
-  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_SPLIT_FP16)
+  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_FP8)
     return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: unknown precision");
   uint64_t m, k, k2, n;
   if (!map_size(a_rows, &m) || !map_size(a_cols, &k) || !map_size(b_rows, &k2) ||
@@ -1763,6 +1756,14 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
     sum_partials_narrow_kernel<<<unsigned(m * n), 256, 0, ctx->stream>>>(part, C2, nchunk, m * n);
     QFX_HIP(hipGetLastError());
     return QFX_OK;
+  }
+  if (precision == QFX_PREC_FP8) {
+    // The quantize kernels read the operands through the maps: every form.
+    if (!qfx_fp8_applies(m, n, k))
+      return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_mapped_c64: no kernel reads this form");
+    QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
+    const qfx_index_map maps[4] = {*a_rows, *a_cols, *b_rows, *b_cols};
+    return qfx_cgemm_fp8(ctx, A2, B2, C2, m, n, k, maps);
   }
   const bool split = precision == QFX_PREC_SPLIT_FP16 || precision == QFX_PREC_SPLIT_BF16 ||
                      precision == QFX_PREC_SPLIT_BF16X3;

@@ -68,3 +68,23 @@ struct QfxLaunchScope {
 
 // Device workspace of at least `bytes` (reused across calls on the stream).
 int qfx_workspace(qfx_context* ctx, size_t bytes, void** out);
+
+// Offset of matrix index idx under a digit map (see qfx_index_map).  The
+// map is a bit permutation, so for i0 a multiple of 2^b and d < 2^b,
+// offset(i0 + d) = offset(i0) + offset(d).
+__device__ __forceinline__ uint64_t map_offset(const qfx_index_map& d, uint64_t idx) {
+  uint64_t off = 0;
+  for (int j = 0; j < d.ndigits; ++j) {
+    const int b = d.bits[j];
+    off += (idx & ((uint64_t(1) << b) - 1)) << d.shift[j];
+    idx >>= b;
+  }
+  return off;
+}
+
+// FP8 (E4M3, per-128x128-tile power-of-two scales) complex GEMM of
+// qfx_fp8.hip: whether the shape is large enough for its tiles, and the
+// launch (maps = {a_rows, a_cols, b_rows, b_cols} or null for row-major).
+bool qfx_fp8_applies(uint64_t m, uint64_t n, uint64_t k);
+int qfx_cgemm_fp8(qfx_context* ctx, const float2* A, const float2* B, float2* C, uint64_t m,
+                  uint64_t n, uint64_t k, const qfx_index_map* maps);

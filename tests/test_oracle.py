@@ -276,3 +276,84 @@ def test_mlp_finite_differences_catch_a_missing_relu_mask():
         fd = mlp.finite_difference(p, X, t, i)
         worst = max(worst, abs(fd - dW1_wrong.reshape(-1)[i]))
     assert worst > 1e-4
+
+
+# ---------------------------------------------------------------- FP8 (E4M3)
+
+def test_e4m3_table_known_values():
+    from oracle import fp8
+    v = fp8.e4m3_values()
+    assert v[0x7e] == 448.0 and v[0x01] == 2.0 ** -9 and v[0x08] == 2.0 ** -6
+    assert v[0x38] == 1.0 and v[0xb8] == -1.0 and np.isnan(v[0x7f]) and np.isnan(v[0xff])
+    pos = np.unique(v[np.isfinite(v) & (v > 0)])
+    assert len(pos) == 126  # 7 subnormals + 15 binades x 8 - the NaN code
+    # every finite value round-trips through the rounding
+    fin = v[np.isfinite(v)]
+    assert np.array_equal(fp8.e4m3_round(fin), fin)
+
+
+def test_e4m3_round_matches_brute_force_nearest_even():
+    """Brute force over the code table: nearest value, ties to the even code."""
+    from oracle import fp8
+    v = fp8.e4m3_values()
+    codes = [c for c in range(256) if np.isfinite(v[c])]
+    vals = np.array([v[c] for c in codes])
+    pos = np.unique(vals[vals >= 0])
+    mids = (pos[1:] + pos[:-1]) / 2
+    rng = np.random.default_rng(5)
+    x = np.concatenate([mids, -mids, rng.uniform(-448, 448, 2000),
+                        rng.standard_normal(2000) * 2.0 ** rng.integers(-12, 8, 2000)])
+    x = x[np.abs(x) <= 448]
+    got = fp8.e4m3_round(x)
+    for xi, gi in zip(x, got):
+        d = np.abs(vals - xi)
+        best = np.flatnonzero(d == d.min())
+        if len(best) > 1:
+            best = [b for b in best if codes[b] % 2 == 0]
+        assert gi == vals[best[0]], (xi, gi)
+
+
+def test_fp8_cgemm_oracle_exact_on_small_integers():
+    """|x| <= 8 integers have at most 3 significant bits: exact at any tile
+    scale, so the FP8 form reproduces the complex product (pins the real
+    embedding, the 2x2 blocks of B and the tiling)."""
+    from oracle import fp8
+    rng = np.random.default_rng(6)
+    a = rng.integers(-8, 9, (200, 150)) + 1j * rng.integers(-8, 9, (200, 150))
+    b = rng.integers(-8, 9, (150, 90)) + 1j * rng.integers(-8, 9, (150, 90))
+    a[:128, :64] *= 2.0 ** 9  # tiles at different scales (sums stay exact in float64)
+    b[64:, :] *= 2.0 ** -7
+    assert np.array_equal(fp8.cgemm_fp8(a, b), a @ b)
+
+
+def test_fp8_quantize_error_bound_and_scale_range():
+    from oracle import fp8
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal((256, 384)) * 2.0 ** rng.integers(-20, 20, (256, 1))
+    q, s = fp8.quantize(x)
+    for ti in range(2):
+        for tj in range(3):
+            blk = x[128 * ti:128 * ti + 128, 128 * tj:128 * tj + 128]
+            scaled = np.abs(blk).max() * 2.0 ** -s[ti, tj]
+            assert 128 <= scaled < 256
+            qb = q[128 * ti:128 * ti + 128, 128 * tj:128 * tj + 128]
+            # relative half-quantum in the normal range, absolute 2^-10 2^s below it
+            bound = np.maximum(np.abs(blk) * 2.0 ** -4, 2.0 ** (s[ti, tj] - 10))
+            assert np.all(np.abs(qb - blk) <= bound)
+
+
+def test_fp8_planted_sign_error_is_caught(monkeypatch):
+    """A Bhat with the -bi block on the wrong row breaks the exact check."""
+    from oracle import fp8
+    good = fp8.real_bhat
+
+    def bad(b):
+        out = good(b)
+        out[0::2, 1::2], out[1::2, 0::2] = out[1::2, 0::2].copy(), out[0::2, 1::2].copy()
+        return out
+
+    monkeypatch.setattr(fp8, "real_bhat", bad)
+    rng = np.random.default_rng(8)
+    a = rng.integers(-8, 9, (64, 40)) + 1j * rng.integers(-8, 9, (64, 40))
+    b = rng.integers(-8, 9, (40, 30)) + 1j * rng.integers(-8, 9, (40, 30))
+    assert not np.array_equal(fp8.cgemm_fp8(a, b), a @ b)
