@@ -9,8 +9,7 @@ contract all 256 cut slices of the corner-inward ordering and deliver the
 Precision variants of the complex GEMMs: fp32 (FP32 MFMA), split_fp16
 (operands scaled by powers of two, FP16 hi+lo, 3 products, FP32
 accumulate), split_bf16x3 (BF16 hi+mid+lo, 6 products), split_bf16
-(hi+mid, 3 products) and fp8 (the low-fidelity mode: E4M3 operands with one
-power-of-two scale per 128x128 tile).  Each is timed; its norm-wise relative
+(hi+mid, 3 products).  Each is timed; its norm-wise relative
 error (and fidelity |<ref|psi>|^2 / (|ref|^2 |psi|^2)) is measured on the
 full job against committed complex128 oracle amplitudes.  `value` is the
 fastest variant whose error is <= 1e-5.
@@ -261,7 +260,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variants", default="fp32,split_fp16,split_bf16x3,split_bf16,fp8")
+    ap.add_argument("--variants", default="fp32,split_fp16,split_bf16x3,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--microbench", default="",
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "

@@ -265,6 +265,31 @@ def test_corner_inward_7x7_ordering_vs_oracle(depth):
     assert relerr([a for _, a in r], [a for _, a in o]) < 1e-5
 
 
+@pytest.mark.skip(reason="engine-level FP8 is disabled: its first run on this workload ended "
+                         "in a GPU memory fault whose cause is not found yet")
+def test_fp8_low_fidelity_mode_vs_complex128_reference():
+    """The FP8 mode on one slice of the 7x7, depth 1+32+1 workload (bonds of
+    dimension 256 after bundling, so the large products run in FP8), against
+    the committed complex128 oracle amplitudes of that slice: amplitudes
+    within a few percent, fidelity |<ref|psi>|^2 / (|ref|^2 |psi|^2) close
+    to 1, and the quantization visibly at work (not an FP32 result)."""
+    ref = json.load(open(os.path.join(os.path.dirname(__file__), "golden",
+                                      "headline_7x7_d32_seed0_slices1.json")))
+    c = ref["circuit"]
+    circ = circuits.random_circuit(c["I"], c["J"], c["depth"], c["seed"])
+    order = circuits.corner_inward_7x7(tuple(ref["cut_values"]))
+    r = dict(run_engine(circ, order, circuits.grid_text(7, 7), "0" * 49, "", "fp8"))
+    got = np.array([r[s] for s, _, _ in ref["amplitudes"]])
+    want = np.array([complex(re, im) for _, re, im in ref["amplitudes"]])
+    fid = abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real)
+    err = relerr(got, want)
+    print("fp8 7x7 d32 one slice: relerr", err, "fidelity", fid)
+    assert 1e-4 < err < FP8_MAX_RELERR and fid > FP8_MIN_FIDELITY
+
+
+FP8_MAX_RELERR, FP8_MIN_FIDELITY = 0.5, 0.75
+
+
 @pytest.mark.parametrize("precision", ["split_fp16", "split_bf16", "fp32"])
 def test_mapped_operands_bit_identical_to_reorders(precision):
     # Reading operands in stored order (no reorder) feeds the same values in
@@ -305,3 +330,12 @@ def test_simulate_with_files(tmp_path):
                         "initial_state": "00000", "final_state": "1100x"})
     assert [s for s, _ in r] == ["11000", "11001"]
     assert abs(r[0][1] - complex(0.10669, 0.04419)) < 1e-5
+
+
+def test_fp8_refused_by_the_engine():
+    """Engine-level FP8 is switched off: simulate reports the error and
+    returns no amplitudes (the reference's error convention)."""
+    circ = circuits.random_circuit(3, 3, 8, 1)
+    with pytest.raises(RuntimeError, match="disabled"):
+        run_engine(circ, circuits.snake_ordering(3, 3), circuits.grid_text(3, 3), "0" * 9,
+                   "0" * 9, "fp8")
