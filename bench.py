@@ -44,10 +44,13 @@ def workload(seed):
     return circ, circuits.corner_inward_7x7(), circuits.grid_text(7, 7)
 
 
+AUTOTUNE = True  # tile autotuner on (the warmup step does the timing of new products)
+
+
 def options(circ, order, grid, precision):
     return {"circuit": circ.splitlines(), "ordering": order.splitlines(),
             "grid": grid.splitlines(), "initial_state": "0" * 49, "final_state": "",
-            "memory_limit": "200GB", "precision": precision}
+            "memory_limit": "200GB", "precision": precision, "autotune": AUTOTUNE}
 
 
 def relerr(got, ref):
@@ -262,6 +265,8 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--variants", default="fp32,split_fp16,split_bf16x3,split_bf16")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="fixed tile rule instead of the tile autotuner")
     ap.add_argument("--microbench", default="",
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
                          "with HIP events through the C-ABI, each --steps times per variant); "
@@ -274,6 +279,8 @@ def main():
                     help="first run one device copy of this many MiB (qfx_copy_kernel), the "
                          "known-size reference for HBM byte counters")
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -354,7 +361,7 @@ def main():
                  "8-pattern family, depth 1+32+1, bond dimension 16"),
         "config": {"workload": "7x7 grid, depth 1+32+1, bond 16, corner-inward ordering, "
                                "2 cuts = 256 slices, 6-qubit open region = 64 amplitudes per step",
-                   "variant": pick, "precision_bar": ERR_BAR,
+                   "variant": pick, "precision_bar": ERR_BAR, "autotune": AUTOTUNE,
                    "passes_bar": pick in ok},
         "variants": variants,
         "roofline": {"bound": "mfma", "kernel": "cgemm_" + pick,

@@ -192,6 +192,27 @@ int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, const qfx_c64*
 int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n, qfx_c128* amps,
                         uint64_t offset);
 
+/* ---- tile autotuner ------------------------------------------------------ */
+/* The split-precision GEMMs run on one of three large tiles (complex rows x
+ * columns): QFX_TILE_256x128, QFX_TILE_128x256 and QFX_TILE_512x32.  By
+ * default a fixed rule picks one from m and n.  With the autotuner on, the
+ * first product of each (precision, operand form, m, n, k) runs once on
+ * every tile that fits (timed with HIP events; this synchronizes the
+ * stream), the fastest is kept in the context and used from then on.  Every
+ * tile gives bit-identical results (same per-element accumulation order),
+ * so tuning changes only speed.  No counterpart in the reference (its
+ * cblas_cgemm has no tile choice). */
+enum qfx_tile { QFX_TILE_256x128 = 1, QFX_TILE_128x256 = 2, QFX_TILE_512x32 = 3 };
+/* Operand forms of the tuning key. */
+enum qfx_form { QFX_FORM_ROW_MAJOR = 0, QFX_FORM_A_ALONG_K = 1, QFX_FORM_A_ALONG_M = 2 };
+int qfx_autotune_enable(qfx_context* ctx, int on);
+/* Entries as 6 values each: precision, form, m, n, k, tile.  Writes at most
+ * `cap` entries; *count = number of entries held. */
+int qfx_autotune_entries(qfx_context* ctx, uint64_t* out, int cap, int* count);
+/* Sets (or, with tile 0, removes) one entry, e.g. from a saved table. */
+int qfx_autotune_set(qfx_context* ctx, int precision, int form, uint64_t m, uint64_t n,
+                     uint64_t k, int tile);
+
 /* ---- the cost model's network ------------------------------------------ */
 /* No counterpart in the reference: the MLP that scores candidate contraction
  * orderings (features from the engine's symbolic plan, labels from its own

@@ -14,6 +14,8 @@ FP32, SPLIT_BF16, FP32_EXACT, SPLIT_BF16X3, SPLIT_FP16, FP8 = 0, 1, 2, 3, 4, 5
 PRECISIONS = {"fp32": FP32, "split_bf16": SPLIT_BF16, "fp32_exact": FP32_EXACT,
               "split_bf16x3": SPLIT_BF16X3, "split_fp16": SPLIT_FP16, "fp8": FP8}
 K_PERMUTE, K_CGEMM, K_GEMV, K_DOT, K_ACCUMULATE, K_COPY, K_CGEMM_SMALL = range(7)
+TILE_256x128, TILE_128x256, TILE_512x32 = 1, 2, 3
+FORM_ROW_MAJOR, FORM_A_ALONG_K, FORM_A_ALONG_M = 0, 1, 2
 KERNEL_CLASSES = ["permute", "cgemm", "gemv", "dot", "accumulate", "copy", "cgemm_small"]
 MLP_IN, MLP_HIDDEN, MLP_PARAMS = 64, 256, 82689
 
@@ -86,6 +88,9 @@ SIGNATURES = {
     "qfx_profile_reset": (_c.c_int, [_vp]),
     "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
                                     _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "qfx_autotune_enable": (_c.c_int, [_vp, _c.c_int]),
+    "qfx_autotune_entries": (_c.c_int, [_vp, _c.POINTER(_u64), _c.c_int, _c.POINTER(_c.c_int)]),
+    "qfx_autotune_set": (_c.c_int, [_vp, _c.c_int, _c.c_int, _u64, _u64, _u64, _c.c_int]),
     "qfx_timer_start": (_c.c_int, [_vp]),
     "qfx_timer_stop": (_c.c_int, [_vp, _c.POINTER(_c.c_float)]),
 }
@@ -194,6 +199,21 @@ class Context:
     def mlp_adam(self, params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         check(load().qfx_mlp_adam(self.handle, params.ptr, grads.ptr, m.ptr, v.ptr, step, lr,
                                   beta1, beta2, eps))
+
+    # ---- tile autotuner
+    def autotune(self, on):
+        check(load().qfx_autotune_enable(self.handle, 1 if on else 0))
+
+    def autotune_entries(self):
+        """[(precision, form, m, n, k, tile)] held by the context."""
+        count = _c.c_int(0)
+        check(load().qfx_autotune_entries(self.handle, None, 0, _c.byref(count)))
+        buf = (_u64 * (6 * max(count.value, 1)))()
+        check(load().qfx_autotune_entries(self.handle, buf, count.value, _c.byref(count)))
+        return [tuple(int(buf[6 * i + j]) for j in range(6)) for i in range(count.value)]
+
+    def autotune_set(self, precision, form, m, n, k, tile):
+        check(load().qfx_autotune_set(self.handle, precision, form, m, n, k, tile))
 
     def cgemm_mapped_ex(self, precision, A, a_rows, a_cols, B, b_rows, b_cols, C, io):
         st = load().qfx_cgemm_mapped_ex_c64(self.handle, precision, A.ptr, ctypes.byref(a_rows),

@@ -6,8 +6,8 @@
 // final_state(s), memory_limit), the same single-state restriction, and
 // errors printed to stderr with [] returned.  Two extra optional keys:
 // "precision" ("fp32" | "split_fp16" | "split_bf16x3" | "split_bf16" |
-// "fp32_exact") and
-// "verbose" (int).
+// "fp32_exact"), "verbose" (int) and "autotune" (bool: time the GEMM tiles
+// on first use and keep the fastest, for this and later calls).
 //
 // The remaining functions expose the host's Tensor / contraction API to the
 // tests; they raise RuntimeError with the reference's message texts.
@@ -119,6 +119,9 @@ void load_input(const py::dict& options, qflex::QflexInput& input) {
   if (options.contains("precision"))
     qflex::global::precision = parse_precision(options["precision"].cast<std::string>());
   if (options.contains("verbose")) qflex::global::verbose = options["verbose"].cast<int>();
+  if (options.contains("autotune"))
+    qflex::qfx_check(qfx_autotune_enable(qflex::device(), options["autotune"].cast<bool>() ? 1 : 0),
+                     "qfx_autotune_enable");
 
   LoadData(options, "grid", input.grid);
   LoadData(options, "circuit", input.circuit);
@@ -492,6 +495,36 @@ PYBIND11_MODULE(qflex, m) {
   m.def("synchronize", [] {
     wrap([&] {
       qflex::qfx_check(qfx_synchronize(qflex::device()), "qfx_synchronize");
+      return 0;
+    });
+  });
+  // Tile autotuner of the device context (see qfx_autotune_* in qfx.h).
+  m.def("set_autotune", [](bool on) {
+    wrap([&] {
+      qflex::qfx_check(qfx_autotune_enable(qflex::device(), on ? 1 : 0), "qfx_autotune_enable");
+      return 0;
+    });
+  });
+  m.def("autotune_table", [] {
+    return wrap([&] {
+      int count = 0;
+      qflex::qfx_check(qfx_autotune_entries(qflex::device(), nullptr, 0, &count),
+                       "qfx_autotune_entries");
+      std::vector<uint64_t> buf(6 * std::size_t(count));
+      qflex::qfx_check(qfx_autotune_entries(qflex::device(), buf.data(), count, &count),
+                       "qfx_autotune_entries");
+      py::list out;
+      for (int i = 0; i < count; ++i)
+        out.append(py::make_tuple(buf[6 * i], buf[6 * i + 1], buf[6 * i + 2], buf[6 * i + 3],
+                                  buf[6 * i + 4], buf[6 * i + 5]));
+      return out;
+    });
+  });
+  m.def("set_autotune_entry", [](int precision, int form, uint64_t m, uint64_t n, uint64_t k,
+                                 int tile) {
+    wrap([&] {
+      qflex::qfx_check(qfx_autotune_set(qflex::device(), precision, form, m, n, k, tile),
+                       "qfx_autotune_set");
       return 0;
     });
   });

@@ -1023,6 +1023,66 @@ uint64_t big_layout_blocks(BigLayout lay, uint64_t m, uint64_t n) {
   return ((m + bm - 1) / bm) * ((n + bn - 1) / bn);
 }
 
+// Tiles that can run a product of this precision and operand form.
+std::vector<BigLayout> layout_candidates(int precision, int am, uint64_t m, uint64_t n,
+                                         uint64_t k) {
+  std::vector<BigLayout> out;
+  if (precision == QFX_PREC_SPLIT_BF16X3) {  // its three planes fit the 256 x 128 tile only
+    out.push_back(kLayout4x2);
+    return out;
+  }
+  for (BigLayout l : {kLayout4x2, kLayout2x4, kLayout8x1}) {
+    if (l == kLayout8x1 && n > 256) continue;  // 32-column tiles for narrow products only
+    if (am != 0 && !split_big_mapped_fits(precision, l, k)) continue;
+    const uint64_t bn = l == kLayout2x4 ? 256 : (l == kLayout8x1 ? 32 : 128);
+    if ((n + bn - 1) / bn > 65535) continue;
+    out.push_back(l);
+  }
+  (void)m;
+  return out;
+}
+
+// The tile of a large-tile product: the fixed rule's choice, or, with the
+// autotuner on, the fastest candidate (each run twice on first use; every
+// candidate writes the same C, so the caller's own launch follows).
+template <class Launch>
+BigLayout tuned_layout(qfx_context* ctx, int precision, int am, uint64_t m, uint64_t n,
+                       uint64_t k, BigLayout def, Launch launch, hipError_t* err) {
+  *err = hipSuccess;
+  if (!ctx->autotune || def == kLayoutNone) return def;
+  const std::array<uint64_t, 5> key{uint64_t(precision), uint64_t(am), m, n, k};
+  auto it = ctx->tuned.find(key);
+  if (it != ctx->tuned.end()) return BigLayout(it->second);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone)
+    return def;  // no timing (and no synchronization) inside a graph capture
+  const std::vector<BigLayout> cands = layout_candidates(precision, am, m, n, k);
+  if (cands.size() < 2) {
+    ctx->tuned[key] = def;
+    return def;
+  }
+  for (auto& e : ctx->tune_ev)
+    if (!e && (*err = hipEventCreate(&e)) != hipSuccess) return def;
+  BigLayout best = def;
+  float best_ms = 1e30f;
+  for (BigLayout l : cands)
+    for (int rep = 0; rep < 2; ++rep) {
+      if ((*err = hipEventRecord(ctx->tune_ev[0], ctx->stream)) != hipSuccess) return def;
+      if ((*err = launch(l)) != hipSuccess) return def;
+      if ((*err = hipEventRecord(ctx->tune_ev[1], ctx->stream)) != hipSuccess) return def;
+      if ((*err = hipEventSynchronize(ctx->tune_ev[1])) != hipSuccess) return def;
+      float ms = 0.f;
+      if ((*err = hipEventElapsedTime(&ms, ctx->tune_ev[0], ctx->tune_ev[1])) != hipSuccess)
+        return def;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = l;
+      }
+    }
+  ctx->tuned[key] = best;
+  return best;
+}
+
 // ------------------------------------------------------------ GEMV / dot
 __device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c) {
   c.x = fmaf(a.x, b.x, c.x);
@@ -1647,7 +1707,16 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   }
   if (layout != kLayoutNone) {
     if ((n + 31) / 32 > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
-    QFX_HIP(launch_split_big_prec<0>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
+    hipError_t te = hipSuccess;
+    const BigLayout use = tuned_layout(
+        ctx, precision, 0, m, n, k, layout,
+        [&](BigLayout l) {
+          return launch_split_big_prec<0>(precision, l, ctx->stream, A2, B2, C2, m, n, k, amax,
+                                          nullptr, nullptr);
+        },
+        &te);
+    QFX_HIP(te);
+    QFX_HIP(launch_split_big_prec<0>(precision, use, ctx->stream, A2, B2, C2, m, n, k, amax,
                                      nullptr, nullptr));
     return QFX_OK;
   }
@@ -1786,18 +1855,66 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
                          a_known ? io->a_max_n : 0, b_known ? io->b_max : nullptr,
                          b_known ? io->b_max_n : 0);
     if (st != QFX_OK) return st;
-    const uint64_t blocks = big_layout_blocks(layout, m, n);
-    if (io && io->c_max && blocks <= uint64_t(std::max(io->c_max_cap, 0))) {
-      cmax = io->c_max;
-      io->c_max_n = int(blocks);
-    }
   }
   const qfx_index_map maps[4] = {*a_rows, *a_cols, *b_rows, *b_cols};
-  if (am == 1)
-    QFX_HIP(launch_split_big_prec<1>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                     maps, cmax));
-  else
-    QFX_HIP(launch_split_big_prec<2>(precision, layout, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                     maps, cmax));
+  // per-block maxima of C for the next product, when the caller's array holds them
+  auto cmax_for = [&](BigLayout l) -> float* {
+    return (precision == QFX_PREC_SPLIT_FP16 && io && io->c_max &&
+            big_layout_blocks(l, m, n) <= uint64_t(std::max(io->c_max_cap, 0)))
+               ? io->c_max
+               : nullptr;
+  };
+  auto launch = [&](BigLayout l) {
+    return am == 1 ? launch_split_big_prec<1>(precision, l, ctx->stream, A2, B2, C2, m, n, k,
+                                              amax, maps, cmax_for(l))
+                   : launch_split_big_prec<2>(precision, l, ctx->stream, A2, B2, C2, m, n, k,
+                                              amax, maps, cmax_for(l));
+  };
+  hipError_t te = hipSuccess;
+  const BigLayout use = tuned_layout(ctx, precision, am, m, n, k, layout, launch, &te);
+  QFX_HIP(te);
+  cmax = cmax_for(use);
+  if (cmax) io->c_max_n = int(big_layout_blocks(use, m, n));
+  QFX_HIP(launch(use));
+  return QFX_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+
+extern "C" int qfx_autotune_enable(qfx_context* ctx, int on) {
+  QFX_CHECK_CTX(ctx);
+  ctx->autotune = on != 0;
+  return QFX_OK;
+}
+
+extern "C" int qfx_autotune_entries(qfx_context* ctx, uint64_t* out, int cap, int* count) {
+  QFX_CHECK_CTX(ctx);
+  int i = 0;
+  for (const auto& kv : ctx->tuned) {
+    if (out && i < cap) {
+      for (int j = 0; j < 5; ++j) out[6 * i + j] = kv.first[j];
+      out[6 * i + 5] = uint64_t(kv.second);
+    }
+    ++i;
+  }
+  if (count) *count = i;
+  return QFX_OK;
+}
+
+extern "C" int qfx_autotune_set(qfx_context* ctx, int precision, int form, uint64_t m,
+                                uint64_t n, uint64_t k, int tile) {
+  QFX_CHECK_CTX(ctx);
+  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_FP8 || form < 0 || form > 2 ||
+      tile < 0 || tile > 3)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_autotune_set: invalid precision, form or tile");
+  const std::array<uint64_t, 5> key{uint64_t(precision), uint64_t(form), m, n, k};
+  if (tile == 0) {
+    ctx->tuned.erase(key);
+    return QFX_OK;
+  }
+  const auto cands = layout_candidates(precision, form, m, n, k);
+  if (std::find(cands.begin(), cands.end(), BigLayout(tile)) == cands.end())
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_autotune_set: the tile cannot run this product");
+  ctx->tuned[key] = tile;
   return QFX_OK;
 }

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <map>
 #include <string>
@@ -42,6 +43,11 @@ struct qfx_context {
   double bytes[QFX_K_NUM_CLASSES] = {};
   double flops[QFX_K_NUM_CLASSES] = {};
   hipEvent_t timer_start = nullptr, timer_stop = nullptr;
+  // Autotuner of the large-tile GEMM layouts: (precision, operand form, m,
+  // n, k) -> layout, measured on first use while enabled.
+  bool autotune = false;
+  std::map<std::array<uint64_t, 5>, int> tuned;
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};
 };
 
 // Error plumbing: every entry point returns a status and records a message.

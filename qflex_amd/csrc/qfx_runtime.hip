@@ -54,6 +54,8 @@ extern "C" int qfx_context_destroy(qfx_context* ctx) {
   for (auto& c : ctx->chunks) hipFree(c.base);
   if (ctx->timer_start) hipEventDestroy(ctx->timer_start);
   if (ctx->timer_stop) hipEventDestroy(ctx->timer_stop);
+  for (auto ev : ctx->tune_ev)
+    if (ev) hipEventDestroy(ev);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return QFX_OK;
