@@ -1471,6 +1471,55 @@ __global__ __launch_bounds__(256) void cgemm_tiny_mn_mapped_partial_kernel(
   }
 }
 
+// k <= 16 and n <= 16 (tall, narrow products such as 2^20 x 16 x 16): one
+// thread per row of C.  B sits in LDS; the thread reads its row of A once;
+// the block's rows of C (one contiguous range of 256 n elements) go through
+// LDS so that the stores are contiguous across the wave.  Each output is the
+// same fmaf chain in increasing k as in cgemm_smallk_kernel (bit-identical).
+template <bool MAPPED>
+__global__ __launch_bounds__(256) void cgemm_rows_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
+    uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
+    qfx_index_map b_cols) {
+  __shared__ float2 Bs[kSmallK][16];
+  __shared__ uint64_t acol[kSmallK];
+  __shared__ float2 Cs[256][17];
+  {
+    const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < K && c < N)
+      v = MAPPED ? B[map_offset(b_rows, r) + map_offset(b_cols, c)] : B[r * N + c];
+    Bs[r][c] = v;
+    if (threadIdx.x < kSmallK)
+      acol[threadIdx.x] = MAPPED && int(threadIdx.x) < K ? map_offset(a_cols, threadIdx.x)
+                                                          : threadIdx.x;
+  }
+  __syncthreads();
+  const uint64_t m0 = uint64_t(blockIdx.x) * 256;
+  const uint64_t row = m0 + threadIdx.x;
+  if (row < M) {
+    const uint64_t abase = MAPPED ? map_offset(a_rows, row) : row * uint64_t(K);
+    float2 acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int kk = 0; kk < kSmallK; ++kk) {
+      if (kk < K) {
+        const float2 a = A[abase + acol[kk]];
+#pragma unroll
+        for (int j = 0; j < 16; ++j) acc[j] = cfma(a, Bs[kk][j], acc[j]);
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) Cs[threadIdx.x][j] = acc[j];
+  }
+  __syncthreads();
+  const uint64_t rows = M - m0 < 256 ? M - m0 : 256;
+  const int total = int(rows) * N;
+  float2* out = C + m0 * uint64_t(N);
+  for (int e = threadIdx.x; e < total; e += 256) out[e] = Cs[e / N][e % N];
+}
+
 unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, 65535))); }
 
 }  // namespace
@@ -1617,7 +1666,10 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     // Small common dimension: exact FP32 on the vector ALUs for every
     // precision (bit-identical to the defined-order scalar kernel).
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
-    if (n <= 16) {
+    if (n <= 16 && (m + 255) / 256 <= 0x7fffffffULL) {
+      cgemm_rows_kernel<false><<<unsigned((m + 255) / 256), 256, 0, ctx->stream>>>(
+          A2, B2, C2, m, int(n), int(k), {}, {}, {}, {});
+    } else if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
                                                                  {}, {}, {}, {}, false, true, k,
@@ -1792,7 +1844,10 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     const bool a_mfast = contiguous(*a_rows, 1) && !contiguous(*a_cols, 3);
     const bool b_nfast = contiguous(*b_cols, 1) || !contiguous(*b_rows, 1);
-    if (n <= 16) {
+    if (n <= 16 && (m + 255) / 256 <= 0x7fffffffULL) {
+      cgemm_rows_kernel<true><<<unsigned((m + 255) / 256), 256, 0, ctx->stream>>>(
+          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols);
+    } else if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(
           A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast, k,
@@ -1814,6 +1869,11 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
     return launch_smallk_splitk(ctx, A2, B2, C2, m, n, k, true, maps, a_mfast, b_nfast);
   }
   if (precision != QFX_PREC_FP32_EXACT && m <= 8 && n <= 8 && k >= 1024) {
+    // Consecutive k go to consecutive lanes: B must be near-contiguous along
+    // k (one row of B within 128 bytes), else the loads are scattered; the
+    // caller then reorders B.
+    if (b_rows->ndigits == 0 || b_rows->shift[0] > 4)
+      return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_mapped_c64: B is not read along k");
     QfxLaunchScope scope(ctx, QFX_K_GEMV, bytes, flops);
     const uint64_t nchunk = (k + kTinyChunk - 1) / kTinyChunk;
     void* ws = nullptr;

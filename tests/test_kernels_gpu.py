@@ -134,6 +134,29 @@ def test_cgemm_reference_known_answer(ctx):
         assert np.array_equal(got.ravel(), np.array([20, 14, 100, 78], dtype=np.complex64))
 
 
+@pytest.mark.parametrize("m,n,k", [(70000, 16, 16), (3000, 7, 11), (1000, 100, 16)])
+def test_small_k_kernels_bit_identical_to_defined_order(ctx, m, n, k):
+    """k <= 16 runs on the vector-ALU kernels (one thread per row for n <= 16,
+    tiles otherwise); appending a zero column to A and a zero row to B makes
+    the FP32_EXACT product go to the scalar kernel (one fmaf chain per output
+    in increasing k): the extra fmaf adds exact zeros, so the results must
+    agree bit for bit."""
+    rng = np.random.default_rng(m + n + k)
+    a = rand_c64(rng, (m, k))
+    b = rand_c64(rng, (k, n))
+    a1 = np.concatenate([a, np.zeros((m, 1), np.complex64)], axis=1)
+    b1 = np.concatenate([b, np.zeros((1, n), np.complex64)], axis=0)
+    A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+    A1, B1, C1 = ctx.upload(a1), ctx.upload(b1), ctx.alloc(m * n * 8)
+    ctx.cgemm(capi.FP32_EXACT, m, n, k, A, B, C)
+    ctx.cgemm(capi.FP32_EXACT, m, n, k + 1, A1, B1, C1)
+    got = ctx.download(C, np.complex64, (m, n))
+    want = ctx.download(C1, np.complex64, (m, n))
+    assert np.array_equal(got, want)
+    for buf in (A, B, C, A1, B1, C1):
+        buf.free()
+
+
 def test_cgemm_exact_integer_inputs_bit_exact(ctx):
     rng = np.random.default_rng(11)
     for (m, n, k) in [(64, 16, 8), (200, 96, 40), (130, 70, 300), (300, 130, 40)]:

@@ -136,6 +136,11 @@ def test_mapped_unsupported_forms(ctx):
     res = run(ctx, [capi.SPLIT_FP16], make_case([16, 16, 16, 16, 2], [16, 2], [8, 16], "left",
                                                 "right", seed=5))
     assert res[capi.SPLIT_FP16] is None
+    # at most 8 x 8 outputs with B stored along n (k's lowest digit at stride
+    # 2^18): the loads along k would be scattered, so the caller reorders B
+    res = run(ctx, [capi.SPLIT_FP16], make_case([2, 2, 2], [16, 16, 16, 16, 16], [2, 2, 2],
+                                                "common", "right", seed=6))
+    assert res[capi.SPLIT_FP16] is None
 
 
 
