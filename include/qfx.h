@@ -192,6 +192,23 @@ int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, const qfx_c64*
 int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n, qfx_c128* amps,
                         uint64_t offset);
 
+/* ---- recorded sequences (HIP graphs) ------------------------------------- */
+/* Between qfx_graph_begin and qfx_graph_end the context's calls are recorded
+ * (HIP stream capture) instead of run; qfx_graph_launch replays the whole
+ * sequence with one launch, on the same buffers.  Meant for a sequence that
+ * repeats with the same pointers, such as the contraction after the cuts of
+ * one slice.  The device workspace is reserved at `workspace_bytes` before
+ * recording and may not grow while recording; calls that synchronize or
+ * allocate device memory from HIP (copies to or from the host,
+ * qfx_synchronize, qfx_profile_*, an arena that must grow) fail while
+ * recording, the tile autotuner times nothing, and profiling is off.  Zero
+ * fills are kernels (no memset nodes).  No counterpart in the reference. */
+typedef struct qfx_graph qfx_graph;
+int qfx_graph_begin(qfx_context* ctx, size_t workspace_bytes);
+int qfx_graph_end(qfx_context* ctx, qfx_graph** out);
+int qfx_graph_launch(qfx_context* ctx, qfx_graph* graph);
+int qfx_graph_destroy(qfx_graph* graph);
+
 /* ---- tile autotuner ------------------------------------------------------ */
 /* The split-precision GEMMs run on one of three large tiles (complex rows x
  * columns): QFX_TILE_256x128, QFX_TILE_128x256 and QFX_TILE_512x32.  By

@@ -88,6 +88,10 @@ SIGNATURES = {
     "qfx_profile_reset": (_c.c_int, [_vp]),
     "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
                                     _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "qfx_graph_begin": (_c.c_int, [_vp, _c.c_size_t]),
+    "qfx_graph_end": (_c.c_int, [_vp, _c.POINTER(_vp)]),
+    "qfx_graph_launch": (_c.c_int, [_vp, _vp]),
+    "qfx_graph_destroy": (_c.c_int, [_vp]),
     "qfx_autotune_enable": (_c.c_int, [_vp, _c.c_int]),
     "qfx_autotune_entries": (_c.c_int, [_vp, _c.POINTER(_u64), _c.c_int, _c.POINTER(_c.c_int)]),
     "qfx_autotune_set": (_c.c_int, [_vp, _c.c_int, _c.c_int, _u64, _u64, _u64, _c.c_int]),
@@ -119,6 +123,22 @@ class QfxError(RuntimeError):
 def check(status):
     if status != 0:
         raise QfxError(f"qfx status {status}: {load().qfx_last_error().decode()}")
+
+
+class Graph:
+    """A recorded sequence of a context's calls (qfx_graph_*)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launch(self):
+        check(load().qfx_graph_launch(self.ctx.handle, self.handle))
+
+    def destroy(self):
+        if self.handle:
+            check(load().qfx_graph_destroy(self.handle))
+            self.handle = None
 
 
 class DeviceBuffer:
@@ -199,6 +219,19 @@ class Context:
     def mlp_adam(self, params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         check(load().qfx_mlp_adam(self.handle, params.ptr, grads.ptr, m.ptr, v.ptr, step, lr,
                                   beta1, beta2, eps))
+
+    def zero(self, buf, nbytes=None):
+        check(load().qfx_zero(self.handle, buf.ptr, buf.nbytes if nbytes is None else nbytes))
+
+    # ---- recorded sequences (HIP graphs)
+    def graph_begin(self, workspace_bytes=0):
+        check(load().qfx_graph_begin(self.handle, workspace_bytes))
+
+    def graph_end(self):
+        """Returns a Graph; replay with graph.launch()."""
+        g = _vp()
+        check(load().qfx_graph_end(self.handle, _c.byref(g)))
+        return Graph(self, g.value)
 
     # ---- tile autotuner
     def autotune(self, on):

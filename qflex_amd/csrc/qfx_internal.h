@@ -45,6 +45,7 @@ struct qfx_context {
   hipEvent_t timer_start = nullptr, timer_stop = nullptr;
   // Autotuner of the large-tile GEMM layouts: (precision, operand form, m,
   // n, k) -> layout, measured on first use while enabled.
+  bool capturing = false;  // between qfx_graph_begin and qfx_graph_end
   bool autotune = false;
   std::map<std::array<uint64_t, 5>, int> tuned;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
@@ -71,6 +72,18 @@ struct QfxLaunchScope {
   QfxLaunchScope(qfx_context* c, int cls, double bytes, double flops);
   ~QfxLaunchScope();
 };
+
+// A recorded sequence of stream work (qfx_graph_begin / qfx_graph_end).
+struct qfx_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  qfx_context* ctx = nullptr;
+};
+
+#define QFX_NOT_CAPTURING(ctx, what)                                                        \
+  if ((ctx)->capturing)                                                                     \
+  return qfx_set_error(QFX_ERR_INVALID, std::string(what) +                                 \
+                                            ": not allowed while a graph is being recorded")
 
 // Device workspace of at least `bytes` (reused across calls on the stream).
 int qfx_workspace(qfx_context* ctx, size_t bytes, void** out);

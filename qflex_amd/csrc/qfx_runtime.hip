@@ -90,6 +90,9 @@ extern "C" int qfx_alloc(qfx_context* ctx, size_t bytes, void** dptr) {
         break;
       }
     }
+    if (!p && ctx->capturing)
+      return qfx_set_error(QFX_ERR_INVALID,
+                           "qfx_alloc: the device arena cannot grow while a graph is recorded");
     if (!p) {
       size_t csz = std::max(ctx->chunk_bytes, sz);
       char* base = nullptr;
@@ -132,6 +135,11 @@ extern "C" int qfx_arena_stats(qfx_context* ctx, size_t* reserved, size_t* in_us
 }
 
 int qfx_workspace(qfx_context* ctx, size_t bytes, void** out) {
+  if (ctx->work_bytes < bytes && ctx->capturing)
+    return qfx_set_error(QFX_ERR_INVALID,
+                         "device workspace of " + std::to_string(bytes) +
+                             " bytes needed while recording a graph; reserve it with "
+                             "qfx_graph_begin's workspace_bytes");
   if (ctx->work_bytes < bytes) {
     if (ctx->work) {
       // The stream may still read the old workspace; it is not freed to HIP,
@@ -163,7 +171,7 @@ static hipEvent_t pool_event(qfx_context* ctx) {
 }
 
 QfxLaunchScope::QfxLaunchScope(qfx_context* c, int cls, double bytes, double flops)
-    : ctx(c), active(c->profiling) {
+    : ctx(c), active(c->profiling && !c->capturing) {
   if (!active) return;
   rec.cls = cls;
   rec.bytes = bytes;
@@ -202,6 +210,7 @@ extern "C" int qfx_profile_enable(qfx_context* ctx, int on) {
 
 extern "C" int qfx_profile_reset(qfx_context* ctx) {
   QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_profile_reset");
   QFX_HIP(hipStreamSynchronize(ctx->stream));
   for (auto& r : ctx->pending) {
     ctx->event_pool.push_back(r.start);
@@ -218,6 +227,7 @@ extern "C" int qfx_profile_reset(qfx_context* ctx) {
 extern "C" int qfx_profile_read(qfx_context* ctx, uint64_t* launches, double* ms,
                                 double* bytes, double* flops) {
   QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_profile_read");
   QFX_HIP(hipStreamSynchronize(ctx->stream));
   for (auto& r : ctx->pending) {
     float t = 0.f;
@@ -259,6 +269,7 @@ extern "C" int qfx_timer_stop(qfx_context* ctx, float* ms) {
 
 extern "C" int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
   QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_copy_h2d");
   if (!bytes) return QFX_OK;
   if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_h2d: null pointer");
   QFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -268,6 +279,7 @@ extern "C" int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t
 
 extern "C" int qfx_copy_d2h(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
   QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_copy_d2h");
   if (!bytes) return QFX_OK;
   if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_d2h: null pointer");
   QFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
@@ -342,6 +354,7 @@ extern "C" int qfx_zero(qfx_context* ctx, void* dst, size_t bytes) {
 
 extern "C" int qfx_synchronize(qfx_context* ctx) {
   QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_synchronize");
   QFX_HIP(hipStreamSynchronize(ctx->stream));
   return QFX_OK;
 }
@@ -370,5 +383,63 @@ extern "C" int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_
   qfx_accumulate_kernel<<<g, 256, 0, ctx->stream>>>(reinterpret_cast<const float2*>(src), n,
                                                      reinterpret_cast<double2*>(amps) + offset);
   QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+
+extern "C" int qfx_graph_begin(qfx_context* ctx, size_t workspace_bytes) {
+  QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_graph_begin");
+  if (workspace_bytes) {
+    void* ws = nullptr;
+    int st = qfx_workspace(ctx, workspace_bytes, &ws);
+    if (st != QFX_OK) return st;
+  }
+  QFX_HIP(hipStreamSynchronize(ctx->stream));
+  QFX_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+  ctx->capturing = true;
+  return QFX_OK;
+}
+
+extern "C" int qfx_graph_end(qfx_context* ctx, qfx_graph** out) {
+  QFX_CHECK_CTX(ctx);
+  if (!ctx->capturing) return qfx_set_error(QFX_ERR_INVALID, "qfx_graph_end: not recording");
+  ctx->capturing = false;
+  hipGraph_t graph = nullptr;
+  QFX_HIP(hipStreamEndCapture(ctx->stream, &graph));
+  if (!out) {
+    hipGraphDestroy(graph);
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_graph_end: null output");
+  }
+  hipGraphExec_t exec = nullptr;
+  hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    hipGraphDestroy(graph);
+    return qfx_hip_error(e, "hipGraphInstantiate");
+  }
+  qfx_graph* g = new qfx_graph;
+  g->graph = graph;
+  g->exec = exec;
+  g->ctx = ctx;
+  *out = g;
+  return QFX_OK;
+}
+
+extern "C" int qfx_graph_launch(qfx_context* ctx, qfx_graph* graph) {
+  QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_graph_launch");
+  if (!graph || graph->ctx != ctx)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_graph_launch: graph of another context");
+  QFX_HIP(hipGraphLaunch(graph->exec, ctx->stream));
+  return QFX_OK;
+}
+
+extern "C" int qfx_graph_destroy(qfx_graph* graph) {
+  if (!graph) return QFX_OK;
+  if (graph->ctx) hipStreamSynchronize(graph->ctx->stream);
+  if (graph->exec) hipGraphExecDestroy(graph->exec);
+  if (graph->graph) hipGraphDestroy(graph->graph);
+  delete graph;
   return QFX_OK;
 }
