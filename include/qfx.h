@@ -120,6 +120,26 @@ int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* dst, int rank
 int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n, uint64_t k,
                   const qfx_c64* A, const qfx_c64* B, qfx_c64* C);
 
+/* Independent products C_i = A_i . B_i (row-major, as qfx_cgemm_c64) of any
+ * shapes.  In the split precisions the products with k > 16, even k and n
+ * and 16-byte aligned operands run as ONE persistent launch of the
+ * 256 x 128 large-tile kernel: the launch's blocks walk the tiles of all
+ * such products (numbered product by product), so many small or uneven
+ * products fill the GPU together; each result is bit-identical to that
+ * product computed alone on the 256 x 128 tile (split FP16: one max-abs
+ * launch per product precedes the grouped launch).  All other products run
+ * one by one through qfx_cgemm_c64.  The C_i must not overlap each other or
+ * any A_j, B_j.  Not allowed while recording a graph.  No counterpart in the
+ * reference (one cblas_cgemm per pairwise contraction, src/tensor.cpp:973). */
+typedef struct qfx_gemm_desc {
+  uint64_t m, n, k;
+  const qfx_c64* A;
+  const qfx_c64* B;
+  qfx_c64* C;
+} qfx_gemm_desc;
+int qfx_cgemm_grouped_c64(qfx_context* ctx, int precision, int count,
+                          const qfx_gemm_desc* descs);
+
 /* Index map of one matrix dimension of a tensor whose dimensions are all
  * powers of two.  The matrix index i is split into digits, least
  * significant first: digit j has bits[j] bits and is the tensor axis whose

@@ -88,6 +88,7 @@ SIGNATURES = {
     "qfx_profile_reset": (_c.c_int, [_vp]),
     "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
                                     _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
+    "qfx_cgemm_grouped_c64": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp]),
     "qfx_graph_begin": (_c.c_int, [_vp, _c.c_size_t]),
     "qfx_graph_end": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "qfx_graph_launch": (_c.c_int, [_vp, _vp]),
@@ -123,6 +124,11 @@ class QfxError(RuntimeError):
 def check(status):
     if status != 0:
         raise QfxError(f"qfx status {status}: {load().qfx_last_error().decode()}")
+
+
+class GemmDesc(_c.Structure):
+    """qfx_gemm_desc: one product of a grouped launch."""
+    _fields_ = [("m", _u64), ("n", _u64), ("k", _u64), ("A", _vp), ("B", _vp), ("C", _vp)]
 
 
 class Graph:
@@ -219,6 +225,14 @@ class Context:
     def mlp_adam(self, params, grads, m, v, step, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         check(load().qfx_mlp_adam(self.handle, params.ptr, grads.ptr, m.ptr, v.ptr, step, lr,
                                   beta1, beta2, eps))
+
+    def cgemm_grouped(self, precision, problems):
+        """problems: [(m, n, k, A, B, C)] with DeviceBuffers; one grouped call."""
+        arr = (GemmDesc * max(len(problems), 1))()
+        for i, (m, n, k, A, B, C) in enumerate(problems):
+            arr[i] = GemmDesc(m, n, k, A.ptr, B.ptr, C.ptr)
+        check(load().qfx_cgemm_grouped_c64(self.handle, precision, len(problems),
+                                           _c.cast(arr, _vp)))
 
     def zero(self, buf, nbytes=None):
         check(load().qfx_zero(self.handle, buf.ptr, buf.nbytes if nbytes is None else nbytes))

@@ -627,12 +627,15 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
 //   AM 2: as 1, but A read along m (a_rows digit 0 at stride 1): each lane
 //         loads rows m, m+1 at one k and stores them as two LDS rows.
 // J = MFMA column tiles per wave (4: 64 complex columns, 2: 32).
+// One output tile (rows m0.., complex columns n0..) of the product; the
+// block's 512 threads all call it.  cmax_slot: where the tile's max |C|
+// goes (or null).
 template <int MODE, int WM, int WN, int J, int AM>
-__global__ __launch_bounds__(512) void cgemm_split_big_kernel(
+__device__ __forceinline__ void split_big_tile(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
-    int N, int K, AmaxRef amax, qfx_index_map a_rows,
-    qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols,
-    float* __restrict__ cmax) {
+    int N, int K, AmaxRef amax, const qfx_index_map& a_rows, const qfx_index_map& a_cols,
+    const qfx_index_map& b_rows, const qfx_index_map& b_cols, float* __restrict__ cmax_slot,
+    const int m0, const int n0) {
   static_assert(WM * WN == 8, "8 waves");
   typedef typename SplitTraits<MODE>::T T;
   typedef typename SplitTraits<MODE>::V V;
@@ -652,8 +655,6 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
   const int wave = tid >> 6;
   const int wm = wave / WN;  // 64 rows each
   const int wn = wave % WN;  // 16*J complex columns each
-  const int m0 = blockIdx.x * kBM;
-  const int n0 = blockIdx.y * kBN;
   float sa = 1.f, sb = 1.f;
 
   // global -> register assignment (f = tid + 512*h, g = tid + 512*h)
@@ -898,7 +899,7 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
           cm = fmaxf(cm, fabsf(v));
         }
       }
-  if (cmax) {
+  if (cmax_slot) {
     // one partial maximum per block, for the split of the GEMM that reads C
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cm = fmaxf(cm, __shfl_xor(cm, off));
@@ -910,8 +911,53 @@ __global__ __launch_bounds__(512) void cgemm_split_big_kernel(
       float b = red[0];
 #pragma unroll
       for (int w = 1; w < 8; ++w) b = fmaxf(b, red[w]);
-      cmax[blockIdx.x + gridDim.x * blockIdx.y] = b;
+      *cmax_slot = b;
     }
+  }
+}
+
+template <int MODE, int WM, int WN, int J, int AM>
+__global__ __launch_bounds__(512) void cgemm_split_big_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
+    int N, int K, AmaxRef amax, qfx_index_map a_rows,
+    qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols,
+    float* __restrict__ cmax) {
+  split_big_tile<MODE, WM, WN, J, AM>(
+      A, B, C, M, N, K, amax, a_rows, a_cols, b_rows, b_cols,
+      cmax ? cmax + blockIdx.x + gridDim.x * blockIdx.y : nullptr, int(blockIdx.x) * 64 * WM,
+      int(blockIdx.y) * 16 * J * WN);
+}
+
+// One problem of a grouped launch: row-major operands, 256 x 128 tiles.
+struct GroupTile {
+  const float2* A;
+  const float2* B;
+  float2* C;
+  int M, N, K, tiles_m;
+  uint64_t first_tile;  // index of the problem's first tile in the launch
+  AmaxRef amax;
+};
+
+// Persistent grouped launch: the blocks walk the tiles of all problems
+// (problems of any shape, tiles numbered problem by problem), each block
+// taking every gridDim.x-th tile.
+template <int MODE>
+__global__ __launch_bounds__(512) void cgemm_split_grouped_kernel(const GroupTile* __restrict__ probs,
+                                                                  int count, uint64_t total) {
+  const qfx_index_map none{};
+  for (uint64_t t = blockIdx.x; t < total; t += gridDim.x) {
+    int lo = 0, hi = count - 1;  // the last problem whose first tile is <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].first_tile <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const GroupTile g = probs[lo];
+    const uint64_t local = t - g.first_tile;
+    const int tm = int(local % uint64_t(g.tiles_m)), tn = int(local / uint64_t(g.tiles_m));
+    split_big_tile<MODE, 4, 2, 4, 0>(g.A, g.B, g.C, g.M, g.N, g.K, g.amax, none, none, none, none,
+                                     nullptr, tm * 256, tn * 128);
+    __syncthreads();  // the next tile's prologue rewrites the LDS
   }
 }
 
@@ -1936,6 +1982,115 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
   cmax = cmax_for(use);
   if (cmax) io->c_max_n = int(big_layout_blocks(use, m, n));
   QFX_HIP(launch(use));
+  return QFX_OK;
+}
+
+// ------------------------------------------------------------ grouped products
+
+extern "C" int qfx_cgemm_grouped_c64(qfx_context* ctx, int precision, int count,
+                                     const qfx_gemm_desc* descs) {
+  QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_cgemm_grouped_c64");
+  if (count < 0 || (count > 0 && !descs))
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_grouped_c64: invalid problem list");
+  if (precision < QFX_PREC_FP32 || precision > QFX_PREC_FP8)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_grouped_c64: unknown precision");
+  const bool split = precision == QFX_PREC_SPLIT_FP16 || precision == QFX_PREC_SPLIT_BF16 ||
+                     precision == QFX_PREC_SPLIT_BF16X3;
+  // Problems for the large tiles go into one launch; the others (other
+  // precisions, k <= 16, odd sizes, unaligned or too large operands) run one
+  // by one exactly as qfx_cgemm_c64 would run them.
+  std::vector<int> grouped;
+  for (int i = 0; i < count; ++i) {
+    const qfx_gemm_desc& d = descs[i];
+    const bool eligible = split && d.m > 0 && d.n > 0 && d.k > uint64_t(kSmallK) &&
+                          d.k % 2 == 0 && d.n % 2 == 0 && d.A && d.B && d.C &&
+                          reinterpret_cast<uintptr_t>(d.A) % 16 == 0 &&
+                          reinterpret_cast<uintptr_t>(d.B) % 16 == 0 &&
+                          big_operands_fit(d.m, d.n, d.k) && d.m <= 0x7fffffffULL &&
+                          d.n <= 0x7fffffffULL;
+    if (eligible) {
+      grouped.push_back(i);
+      continue;
+    }
+    const int st = qfx_cgemm_c64(ctx, precision, d.m, d.n, d.k, d.A, d.B, d.C);
+    if (st != QFX_OK) return st;
+  }
+  if (grouped.empty()) return QFX_OK;
+  double flops = 0, bytes = 0;
+  size_t nparts = 0;
+  std::vector<std::array<int, 2>> blocks(grouped.size(), {0, 0});
+  for (size_t g = 0; g < grouped.size(); ++g) {
+    const qfx_gemm_desc& d = descs[grouped[g]];
+    flops += 8.0 * double(d.m) * double(d.n) * double(d.k);
+    bytes += 8.0 * (double(d.m) * d.k + double(d.k) * d.n + double(d.m) * d.n);
+    if (precision == QFX_PREC_SPLIT_FP16) {
+      blocks[g][0] = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (d.m * d.k + 16383) / 16384)));
+      blocks[g][1] = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (d.k * d.n + 16383) / 16384)));
+      nparts += size_t(blocks[g][0] + blocks[g][1]);
+    }
+  }
+  QfxLaunchScope scope(ctx, QFX_K_CGEMM, bytes, flops);
+  const size_t part_bytes = (nparts * sizeof(float) + 255) / 256 * 256;
+  const size_t desc_bytes = grouped.size() * sizeof(GroupTile);
+  void* ws = nullptr;
+  int st = qfx_workspace(ctx, part_bytes + desc_bytes, &ws);
+  if (st != QFX_OK) return st;
+  float* part = static_cast<float*>(ws);
+  GroupTile* dev = reinterpret_cast<GroupTile*>(static_cast<char*>(ws) + part_bytes);
+  // host staging of the problem list: the previous grouped call's copy
+  // must have been read before the staging buffer is rewritten
+  if (!ctx->group_ev) QFX_HIP(hipEventCreateWithFlags(&ctx->group_ev, hipEventDisableTiming));
+  else QFX_HIP(hipEventSynchronize(ctx->group_ev));
+  ctx->group_host.resize(desc_bytes);
+  GroupTile* host = reinterpret_cast<GroupTile*>(ctx->group_host.data());
+  uint64_t tile = 0;
+  size_t off = 0;
+  for (size_t g = 0; g < grouped.size(); ++g) {
+    const qfx_gemm_desc& d = descs[grouped[g]];
+    const float2* A2 = reinterpret_cast<const float2*>(d.A);
+    const float2* B2 = reinterpret_cast<const float2*>(d.B);
+    GroupTile t{};
+    t.A = A2;
+    t.B = B2;
+    t.C = reinterpret_cast<float2*>(d.C);
+    t.M = int(d.m);
+    t.N = int(d.n);
+    t.K = int(d.k);
+    t.tiles_m = int((d.m + 255) / 256);
+    t.first_tile = tile;
+    tile += uint64_t(t.tiles_m) * ((d.n + 127) / 128);
+    if (precision == QFX_PREC_SPLIT_FP16) {
+      const int ba = blocks[g][0], bb = blocks[g][1];
+      amax_partial_kernel<<<unsigned(ba + bb), 256, 0, ctx->stream>>>(A2, d.m * d.k, B2,
+                                                                      d.k * d.n, ba, part + off);
+      t.amax = AmaxRef{part + off, ba, part + off + ba, bb};
+      off += size_t(ba + bb);
+    }
+    host[g] = t;
+  }
+  QFX_HIP(hipGetLastError());
+  QFX_HIP(hipMemcpyAsync(dev, host, desc_bytes, hipMemcpyHostToDevice, ctx->stream));
+  QFX_HIP(hipEventRecord(ctx->group_ev, ctx->stream));
+  if (ctx->cus <= 0) {
+    int cus = 0;
+    QFX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    ctx->cus = std::max(cus, 1);
+  }
+  const unsigned grid = unsigned(std::min<uint64_t>(tile, uint64_t(ctx->cus)));
+  auto go = [&](auto mode) -> hipError_t {
+    constexpr int MODE = decltype(mode)::value;
+    static hipError_t e = hipFuncSetAttribute(
+        reinterpret_cast<const void*>(&cgemm_split_grouped_kernel<MODE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(split_big_smem<MODE, 4, 2, 4>()));
+    if (e != hipSuccess) return e;
+    cgemm_split_grouped_kernel<MODE><<<grid, 512, split_big_smem<MODE, 4, 2, 4>(), ctx->stream>>>(
+        dev, int(grouped.size()), tile);
+    return hipGetLastError();
+  };
+  if (precision == QFX_PREC_SPLIT_BF16X3) QFX_HIP(go(std::integral_constant<int, 0>{}));
+  else if (precision == QFX_PREC_SPLIT_BF16) QFX_HIP(go(std::integral_constant<int, 1>{}));
+  else QFX_HIP(go(std::integral_constant<int, 2>{}));
   return QFX_OK;
 }
 

@@ -46,6 +46,11 @@ struct qfx_context {
   // Autotuner of the large-tile GEMM layouts: (precision, operand form, m,
   // n, k) -> layout, measured on first use while enabled.
   bool capturing = false;  // between qfx_graph_begin and qfx_graph_end
+  // Grouped products: host staging of the problem list, the event after its
+  // copy to the device, and the compute-unit count (grid of the launch).
+  std::vector<unsigned char> group_host;
+  hipEvent_t group_ev = nullptr;
+  int cus = 0;
   bool autotune = false;
   std::map<std::array<uint64_t, 5>, int> tuned;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};

@@ -56,6 +56,7 @@ extern "C" int qfx_context_destroy(qfx_context* ctx) {
   if (ctx->timer_stop) hipEventDestroy(ctx->timer_stop);
   for (auto ev : ctx->tune_ev)
     if (ev) hipEventDestroy(ev);
+  if (ctx->group_ev) hipEventDestroy(ctx->group_ev);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return QFX_OK;

@@ -1,0 +1,95 @@
+"""GPU tests of grouped products (qfx_cgemm_grouped_c64): products of uneven
+shapes in one persistent launch give the complex128 result to the
+precision's tolerance, and bit for bit what each product gives alone on the
+256 x 128 tile; products outside the large tiles run one by one."""
+import time
+
+import numpy as np
+import pytest
+
+from qflex_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+TOL = {capi.SPLIT_FP16: 5e-6, capi.SPLIT_BF16: 3e-4, capi.SPLIT_BF16X3: 5e-6, capi.FP32: 2e-6}
+
+SHAPES = [(16384, 256, 256), (33000, 130, 74), (300, 130, 74), (64, 64, 32), (1000, 20, 16),
+          (512, 512, 48), (7, 300, 128), (2048, 18, 40)]
+
+
+def rand_c(rng, shape):
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def make(ctx, rng, shapes):
+    probs, host = [], []
+    for (m, n, k) in shapes:
+        a, b = rand_c(rng, (m, k)), rand_c(rng, (k, n))
+        host.append((a, b))
+        probs.append((m, n, k, ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)))
+    return probs, host
+
+
+def free(probs):
+    for p in probs:
+        for buf in p[3:]:
+            buf.free()
+
+
+@pytest.mark.parametrize("prec", [capi.SPLIT_FP16, capi.SPLIT_BF16, capi.SPLIT_BF16X3, capi.FP32])
+def test_grouped_uneven_shapes(ctx, prec):
+    rng = np.random.default_rng(prec + 40)
+    probs, host = make(ctx, rng, SHAPES)
+    ctx.cgemm_grouped(prec, probs)
+    got = [ctx.download(p[5], np.complex64, (p[0], p[1])) for p in probs]
+    for (m, n, k), (a, b), g in zip(SHAPES, host, got):
+        ref = a.astype(np.complex128) @ b.astype(np.complex128)
+        err = np.linalg.norm(g - ref) / np.linalg.norm(ref)
+        assert err <= TOL[prec], ((m, n, k), prec, err)
+    # products large enough for the 256 x 128 tile alone give the same bits
+    for i in (0, 1):
+        m, n, k, A, B, C = probs[i]
+        single = ctx.alloc(m * n * 8)
+        ctx.cgemm(prec, m, n, k, A, B, single)
+        assert np.array_equal(ctx.download(single, np.complex64, (m, n)), got[i]), (i, prec)
+        single.free()
+    free(probs)
+
+
+def test_grouped_many_small_slices(ctx):
+    """256 products of one slice-like shape: one launch, against one call
+    each (same kernel class for products below the large-tile block count)."""
+    rng = np.random.default_rng(7)
+    shapes = [(256, 128, 64)] * 256
+    probs, host = make(ctx, rng, shapes)
+    ctx.cgemm_grouped(capi.SPLIT_FP16, probs)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.cgemm_grouped(capi.SPLIT_FP16, probs)
+    ctx.sync()
+    t_grouped = time.perf_counter() - t0
+    got = [ctx.download(p[5], np.complex64, (256, 128)) for p in probs]
+    t0 = time.perf_counter()
+    for (m, n, k, A, B, C) in probs:
+        ctx.cgemm(capi.SPLIT_FP16, m, n, k, A, B, C)
+    ctx.sync()
+    t_single = time.perf_counter() - t0
+    print("256 x (256x128x64): grouped %.3f ms, one by one %.3f ms" % (1e3 * t_grouped,
+                                                                       1e3 * t_single))
+    for (a, b), g in zip(host[:8], got[:8]):
+        ref = a.astype(np.complex128) @ b.astype(np.complex128)
+        assert np.linalg.norm(g - ref) / np.linalg.norm(ref) <= TOL[capi.SPLIT_FP16]
+    free(probs)
+
+
+def test_grouped_empty_and_refused_while_recording(ctx):
+    ctx.cgemm_grouped(capi.SPLIT_FP16, [])
+    rng = np.random.default_rng(9)
+    probs, _ = make(ctx, rng, [(512, 256, 64)])
+    ctx.graph_begin()
+    try:
+        with pytest.raises(RuntimeError, match="recorded"):
+            ctx.cgemm_grouped(capi.SPLIT_FP16, probs)
+    finally:
+        ctx.graph_end().destroy()
+    free(probs)
