@@ -329,6 +329,9 @@ def main():
                         "gbps": p["bytes"] / max(p["ms"], 1e-9) / 1e6},
             "class_ms": {c: round(v["ms"], 3) for c, v in prof.items() if v["launches"]},
             "flops_per_step": stats["flops"], "slices": stats["slices"],
+            # ContractGrid alone (device-synchronized wall time of the last
+            # timed step); the rest of a step is the network build and flatten
+            "contraction_ms": 1e3 * stats["seconds"],
             "_prof": prof,
         }
         log(f"[bench] {prec}: {variants[prec]['amplitudes_per_s']:.2f} amp/s, "

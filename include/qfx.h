@@ -99,6 +99,12 @@ int qfx_copy_d2h(qfx_context* ctx, void* dst, const void* src, size_t bytes);
 /* Replaces std::copy of Tensor::_copy / project (src/tensor.cpp:107-139,
  * :307-315). */
 int qfx_copy_d2d(qfx_context* ctx, void* dst, const void* src, size_t bytes);
+/* Projection of a tensor viewed as [pre][dim][post] onto index values:
+ * dst[a][j][r] = src[a][picks[j]][r], j < npicks (one launch per 64 values).
+ * Replaces the copy loop of Tensor::project (src/tensor.cpp:280-316) and its
+ * use on cut indices in ContractGrid (src/contraction_utils.cpp:296-310). */
+int qfx_project_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* dst, uint64_t pre,
+                    uint64_t dim, uint64_t post, uint64_t npicks, const uint64_t* picks);
 /* Zero-fills with a kernel (graph-capture safe). */
 int qfx_zero(qfx_context* ctx, void* dst, size_t bytes);
 int qfx_synchronize(qfx_context* ctx);
@@ -187,6 +193,12 @@ typedef struct qfx_scale_io {
 } qfx_scale_io;
 
 /* qfx_cgemm_mapped_c64 with the scale chain above (io may be NULL). */
+/* Partial maxima of |re|, |im| of X[0..n) (at most 256 values, written to
+ * device memory `out`, *count of them) in the form the producing GEMM's
+ * qfx_scale_io.c_max has: a caller that reuses an operand (a grid tensor in
+ * every slice) computes them once and passes them as a_max / b_max. */
+int qfx_amax_partials_c64(qfx_context* ctx, const qfx_c64* X, uint64_t n, float* out, int cap,
+                          int* count);
 int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qfx_c64* A,
                             const qfx_index_map* a_rows, const qfx_index_map* a_cols,
                             const qfx_c64* B, const qfx_index_map* b_rows,

@@ -89,6 +89,8 @@ SIGNATURES = {
     "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
                                     _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
     "qfx_cgemm_grouped_c64": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp]),
+    "qfx_project_c64": (_c.c_int, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _c.POINTER(_u64)]),
+    "qfx_amax_partials_c64": (_c.c_int, [_vp, _vp, _u64, _vp, _c.c_int, _c.POINTER(_c.c_int)]),
     "qfx_graph_begin": (_c.c_int, [_vp, _c.c_size_t]),
     "qfx_graph_end": (_c.c_int, [_vp, _c.POINTER(_vp)]),
     "qfx_graph_launch": (_c.c_int, [_vp, _vp]),
@@ -233,6 +235,18 @@ class Context:
             arr[i] = GemmDesc(m, n, k, A.ptr, B.ptr, C.ptr)
         check(load().qfx_cgemm_grouped_c64(self.handle, precision, len(problems),
                                            _c.cast(arr, _vp)))
+
+    def project(self, src, dst, pre, dim, post, picks):
+        """dst[a][j][r] = src[a][picks[j]][r] (complex64 tensors as [pre][dim][post])."""
+        arr = (_u64 * max(len(picks), 1))(*picks)
+        check(load().qfx_project_c64(self.handle, src.ptr, dst.ptr, pre, dim, post, len(picks),
+                                     arr))
+
+    def amax_partials(self, X, n, out, cap=256):
+        """Partial maxima of |re|, |im| of X[0..n) into device floats `out`; returns their count."""
+        count = _c.c_int(0)
+        check(load().qfx_amax_partials_c64(self.handle, X.ptr, n, out.ptr, cap, _c.byref(count)))
+        return count.value
 
     def zero(self, buf, nbytes=None):
         check(load().qfx_zero(self.handle, buf.ptr, buf.nbytes if nbytes is None else nbytes))

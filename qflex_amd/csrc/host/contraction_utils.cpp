@@ -442,6 +442,28 @@ bool ContractionData::try_mapped(const s_type* a, const std::vector<std::string>
   return true;
 }
 
+void ContractionData::grid_operand_maxima(const View& v, const float** out, int* count) {
+  bool in_grid = false;
+  for (const auto& r : grid_ranges_)
+    if (v.ptr >= r.first && v.ptr + v.size() <= r.second) in_grid = true;
+  if (!in_grid) return;  // patches change from slice to slice: no caching
+  const auto key = std::make_pair(static_cast<const s_type*>(v.ptr), v.size());
+  auto it = grid_amax_.find(key);
+  if (it == grid_amax_.end()) {
+    const int cap = 256;
+    grid_amax_store_.emplace_back(std::vector<std::string>{""},
+                                  std::vector<std::size_t>{std::size_t(cap / 2)});
+    float* arr = reinterpret_cast<float*>(grid_amax_store_.back().data());
+    int n = 0;
+    qfx_check(qfx_amax_partials_c64(device(), reinterpret_cast<const qfx_c64*>(v.ptr), v.size(),
+                                    arr, cap, &n),
+              "qfx_amax_partials_c64");
+    it = grid_amax_.emplace(key, std::make_pair(static_cast<const float*>(arr), n)).first;
+  }
+  *out = it->second.first;
+  *count = it->second.second;
+}
+
 ContractionData::View ContractionData::contract_pair(const View& A, const View& B, s_type* out) {
   std::vector<std::string> left = _vector_subtraction(A.indices, B.indices);
   std::vector<std::string> right = _vector_subtraction(B.indices, A.indices);
@@ -472,6 +494,10 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
   io.a_max_n = A.amax_n;
   io.b_max = B.amax;
   io.b_max_n = B.amax_n;
+  if (global::precision == QFX_PREC_SPLIT_FP16) {
+    if (io.a_max_n == 0) grid_operand_maxima(A, &io.a_max, &io.a_max_n);
+    if (io.b_max_n == 0) grid_operand_maxima(B, &io.b_max, &io.b_max_n);
+  }
   auto slot = amax_slots_.find(out);
   if (slot != amax_slots_.end()) {
     io.c_max = slot->second.first;
@@ -564,12 +590,12 @@ ContractionData::View ContractionData::project_view(const View& v, const std::st
     return out;
   }
   if (buffer == nullptr) throw ERROR_MSG("No buffer for projection of index ", index, ".");
-  // out[a][k][rest] = v[a][picks[k]][rest]
-  for (std::size_t a = 0; a < pre; ++a)
-    for (std::size_t k = 0; k < picks.size(); ++k)
-      qfx_check(qfx_copy_d2d(device(), buffer + (a * picks.size() + k) * post,
-                             v.ptr + (a * dim + picks[k]) * post, post * sizeof(s_type)),
-                "qfx_copy_d2d");
+  // out[a][k][rest] = v[a][picks[k]][rest], one launch
+  std::vector<uint64_t> pk(picks.begin(), picks.end());
+  qfx_check(qfx_project_c64(device(), reinterpret_cast<const qfx_c64*>(v.ptr),
+                            reinterpret_cast<qfx_c64*>(buffer), pre, dim, post, pk.size(),
+                            pk.data()),
+            "qfx_project_c64");
   out.ptr = buffer;
   return out;
 }
@@ -737,6 +763,12 @@ void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
       }
     }
     patches_.clear();
+    grid_amax_.clear();
+    grid_amax_store_.clear();
+    grid_ranges_.clear();
+    for (const auto& row : *tensor_grid_)
+      for (const auto& t : row)
+        grid_ranges_.push_back({t.data(), t.data() + t.capacity()});
     ContractionStats fresh;
     fresh.planned_bytes = stats_.planned_bytes;
     stats_ = fresh;

@@ -18,6 +18,8 @@
 #define QFLEX_AMD_CONTRACTION_UTILS_H_
 
 #include <chrono>
+#include <deque>
+#include <map>
 #include <complex>
 #include <list>
 #include <optional>
@@ -167,6 +169,13 @@ class ContractionData {
   // tensor a GEMM writes there (pointer, capacity in floats)
   std::vector<Tensor> amax_store_;
   std::unordered_map<const s_type*, std::pair<float*, int>> amax_slots_;
+  // split-FP16 partial maxima of grid-tensor operands (constant during one
+  // ContractGrid), computed at their first use: (pointer, size) -> (array,
+  // count); the address ranges of the grid tensors' buffers
+  std::map<std::pair<const s_type*, std::size_t>, std::pair<const float*, int>> grid_amax_;
+  std::deque<Tensor> grid_amax_store_;
+  std::vector<std::pair<const s_type*, const s_type*>> grid_ranges_;
+  void grid_operand_maxima(const View& v, const float** out, int* count);
   std::unordered_map<std::string, std::size_t> scratch_map_;
   std::chrono::steady_clock::time_point start_;
   std::unordered_map<std::string, std::size_t> patch_max_size_;

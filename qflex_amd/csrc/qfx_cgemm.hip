@@ -1693,6 +1693,25 @@ static int launch_amax(qfx_context* ctx, const float2* A, uint64_t na, const flo
   return QFX_OK;
 }
 
+extern "C" int qfx_amax_partials_c64(qfx_context* ctx, const qfx_c64* X, uint64_t n,
+                                     float* out, int cap, int* count) {
+  QFX_CHECK_CTX(ctx);
+  if (count) *count = 0;
+  if (n == 0) return QFX_OK;
+  if (!X || !out) return qfx_set_error(QFX_ERR_INVALID, "qfx_amax_partials_c64: null pointer");
+  const int b = int(std::min<uint64_t>(256, std::max<uint64_t>(1, (n + 16383) / 16384)));
+  if (b > cap)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_amax_partials_c64: " + std::to_string(b) +
+                                              " partial maxima do not fit in " +
+                                              std::to_string(cap));
+  QfxLaunchScope scope(ctx, QFX_K_COPY, 8.0 * n, 0);
+  amax_partial_kernel<<<unsigned(b), 256, 0, ctx->stream>>>(
+      reinterpret_cast<const float2*>(X), n, nullptr, 0, b, out);
+  QFX_HIP(hipGetLastError());
+  if (count) *count = b;
+  return QFX_OK;
+}
+
 extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n,
                              uint64_t k, const qfx_c64* A, const qfx_c64* B, qfx_c64* C) {
   QFX_CHECK_CTX(ctx);

@@ -326,6 +326,25 @@ extern "C" int qfx_copy_d2d(qfx_context* ctx, void* dst, const void* src, size_t
   return QFX_OK;
 }
 
+struct QfxPicks {
+  uint32_t v[64];
+};
+
+// dst[a][j0 + j][r] = src[a][picks[j]][r] for a < pre, j < npk, r < post
+// (elements of 8 bytes); dst has `nall` entries along j.
+__global__ void qfx_project_kernel(const float2* __restrict__ src, float2* __restrict__ dst,
+                                   uint64_t pre, uint64_t dim, uint64_t post, uint32_t npk,
+                                   uint64_t j0, uint64_t nall, QfxPicks picks) {
+  const uint64_t total = pre * npk * post;
+  uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  for (; e < total; e += stride) {
+    const uint64_t r = e % post, q = e / post;
+    const uint64_t j = q % npk, a = q / npk;
+    dst[(a * nall + j0 + j) * post + r] = src[(a * dim + picks.v[j]) * post + r];
+  }
+}
+
 __global__ void qfx_zero_kernel(unsigned char* __restrict__ dst, uint64_t n) {
   uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
@@ -442,5 +461,32 @@ extern "C" int qfx_graph_destroy(qfx_graph* graph) {
   if (graph->exec) hipGraphExecDestroy(graph->exec);
   if (graph->graph) hipGraphDestroy(graph->graph);
   delete graph;
+  return QFX_OK;
+}
+
+// ---------------------------------------------------------------- projections
+
+extern "C" int qfx_project_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* dst, uint64_t pre,
+                               uint64_t dim, uint64_t post, uint64_t npicks,
+                               const uint64_t* picks) {
+  QFX_CHECK_CTX(ctx);
+  if (pre == 0 || post == 0 || npicks == 0) return QFX_OK;
+  if (!src || !dst || !picks)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_project_c64: null pointer");
+  for (uint64_t j = 0; j < npicks; ++j)
+    if (picks[j] >= dim)
+      return qfx_set_error(QFX_ERR_INVALID, "qfx_project_c64: index value " +
+                                                std::to_string(picks[j]) + " outside [0, " +
+                                                std::to_string(dim) + ")");
+  QfxLaunchScope scope(ctx, QFX_K_COPY, 16.0 * pre * npicks * post, 0);
+  for (uint64_t j0 = 0; j0 < npicks; j0 += 64) {
+    const uint32_t npk = uint32_t(std::min<uint64_t>(64, npicks - j0));
+    QfxPicks pk{};
+    for (uint32_t j = 0; j < npk; ++j) pk.v[j] = uint32_t(picks[j0 + j]);
+    qfx_project_kernel<<<grid_for(pre * npk * post, 256), 256, 0, ctx->stream>>>(
+        reinterpret_cast<const float2*>(src), reinterpret_cast<float2*>(dst), pre, dim, post, npk,
+        j0, npicks, pk);
+  }
+  QFX_HIP(hipGetLastError());
   return QFX_OK;
 }

@@ -226,3 +226,32 @@ def test_profile_counts_launches(ctx):
     assert prof["cgemm"]["launches"] == 1 and prof["permute"]["launches"] == 1
     assert prof["cgemm"]["flops"] == 8.0 * 1024 * 512 * 64
     assert prof["cgemm"]["ms"] > 0
+
+
+@pytest.mark.parametrize("pre,dim,post,picks", [(1, 16, 4096, [3]), (16, 16, 256, [0]),
+                                                  (4, 8, 33, [7, 0, 5]), (3, 130, 5, list(range(130)))])
+def test_project_gathers_index_values(ctx, pre, dim, post, picks):
+    """Tensor::project onto index values (src/tensor.cpp:280-316) in one launch."""
+    rng = np.random.default_rng(pre + dim + post)
+    x = rand_c64(rng, (pre, dim, post))
+    X, Y = ctx.upload(x), ctx.alloc(pre * len(picks) * post * 8)
+    ctx.project(X, Y, pre, dim, post, picks)
+    got = ctx.download(Y, np.complex64, (pre, len(picks), post))
+    assert np.array_equal(got, x[:, picks, :])
+    with pytest.raises(RuntimeError):
+        ctx.project(X, Y, pre, dim, post, [dim])
+    X.free(); Y.free()
+
+
+@pytest.mark.parametrize("n", [1, 1000, 16384 * 5 + 3, 1 << 23])
+def test_amax_partials(ctx, n):
+    rng = np.random.default_rng(n)
+    x = rand_c64(rng, (n,))
+    x[n // 2] = 7.5 - 30j
+    X, out = ctx.upload(x), ctx.alloc(256 * 4)
+    cnt = ctx.amax_partials(X, n, out)
+    parts = ctx.download(out, np.float32, (256,))[:cnt]
+    assert 1 <= cnt <= 256
+    assert parts.max() == np.float32(30.0)
+    assert parts.max() == max(np.abs(x.real).max(), np.abs(x.imag).max())
+    X.free(); out.free()
