@@ -89,6 +89,7 @@ SIGNATURES = {
     "qfx_profile_read": (_c.c_int, [_vp, _c.POINTER(_u64), _c.POINTER(_c.c_double),
                                     _c.POINTER(_c.c_double), _c.POINTER(_c.c_double)]),
     "qfx_cgemm_grouped_c64": (_c.c_int, [_vp, _c.c_int, _c.c_int, _vp]),
+    "qfx_copy_h2d_async": (_c.c_int, [_vp, _vp, _vp, _c.c_size_t]),
     "qfx_project_c64": (_c.c_int, [_vp, _vp, _vp, _u64, _u64, _u64, _u64, _c.POINTER(_u64)]),
     "qfx_amax_partials_c64": (_c.c_int, [_vp, _vp, _u64, _vp, _c.c_int, _c.POINTER(_c.c_int)]),
     "qfx_graph_begin": (_c.c_int, [_vp, _c.c_size_t]),
@@ -184,6 +185,11 @@ class Context:
         buf = self.alloc(arr.nbytes)
         check(load().qfx_copy_h2d(self.handle, buf.ptr, arr.ctypes.data, arr.nbytes))
         return buf
+
+    def upload_async(self, buf, arr):
+        """Staged host-to-device copy of `arr` into `buf` (returns before the copy is done)."""
+        arr = np.ascontiguousarray(arr)
+        check(load().qfx_copy_h2d_async(self.handle, buf.ptr, arr.ctypes.data, arr.nbytes))
 
     def download(self, buf, dtype, shape):
         out = np.empty(shape, dtype=dtype)

@@ -112,8 +112,8 @@ Tensor::Tensor(const std::vector<std::string>& indices,
   }
   _capacity = this_size;
   _data = device_alloc(_capacity);
-  qfx_check(qfx_copy_h2d(device(), _data, data.data(), this_size * sizeof(s_type)),
-            "qfx_copy_h2d");
+  qfx_check(qfx_copy_h2d_async(device(), _data, data.data(), this_size * sizeof(s_type)),
+            "qfx_copy_h2d_async");
 }
 
 Tensor::Tensor(const Tensor& other) { _copy(other); }

@@ -46,6 +46,10 @@ struct qfx_context {
   // Autotuner of the large-tile GEMM layouts: (precision, operand form, m,
   // n, k) -> layout, measured on first use while enabled.
   bool capturing = false;  // between qfx_graph_begin and qfx_graph_end
+  // Pinned staging area of qfx_copy_h2d_async: filled from offset
+  // stage_off on, rewound only after the stream has been synchronized.
+  unsigned char* stage = nullptr;
+  size_t stage_off = 0;
   // Grouped products: host staging of the problem list, the event after its
   // copy to the device, and the compute-unit count (grid of the launch).
   std::vector<unsigned char> group_host;

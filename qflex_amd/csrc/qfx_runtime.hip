@@ -57,6 +57,7 @@ extern "C" int qfx_context_destroy(qfx_context* ctx) {
   for (auto ev : ctx->tune_ev)
     if (ev) hipEventDestroy(ev);
   if (ctx->group_ev) hipEventDestroy(ctx->group_ev);
+  if (ctx->stage) hipHostFree(ctx->stage);
   hipStreamDestroy(ctx->stream);
   delete ctx;
   return QFX_OK;
@@ -275,6 +276,33 @@ extern "C" int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t
   if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_h2d: null pointer");
   QFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
   QFX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stage_off = 0;
+  return QFX_OK;
+}
+
+constexpr size_t kStageBytes = size_t(16) << 20;
+
+extern "C" int qfx_copy_h2d_async(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
+  QFX_CHECK_CTX(ctx);
+  QFX_NOT_CAPTURING(ctx, "qfx_copy_h2d_async");
+  if (!bytes) return QFX_OK;
+  if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_h2d_async: null pointer");
+  if (bytes > kStageBytes / 4) return qfx_copy_h2d(ctx, dst, src, bytes);
+  if (!ctx->stage) {
+    void* p = nullptr;
+    QFX_HIP(hipHostMalloc(&p, kStageBytes, hipHostMallocDefault));
+    ctx->stage = static_cast<unsigned char*>(p);
+    ctx->stage_off = 0;
+  }
+  size_t off = (ctx->stage_off + 255) & ~size_t(255);
+  if (off + bytes > kStageBytes) {
+    // every earlier staged copy has to be done before the area is reused
+    QFX_HIP(hipStreamSynchronize(ctx->stream));
+    off = 0;
+  }
+  std::memcpy(ctx->stage + off, src, bytes);
+  QFX_HIP(hipMemcpyAsync(dst, ctx->stage + off, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ctx->stage_off = off + bytes;
   return QFX_OK;
 }
 
@@ -285,6 +313,7 @@ extern "C" int qfx_copy_d2h(qfx_context* ctx, void* dst, const void* src, size_t
   if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_d2h: null pointer");
   QFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
   QFX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stage_off = 0;
   return QFX_OK;
 }
 
@@ -376,6 +405,7 @@ extern "C" int qfx_synchronize(qfx_context* ctx) {
   QFX_CHECK_CTX(ctx);
   QFX_NOT_CAPTURING(ctx, "qfx_synchronize");
   QFX_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stage_off = 0;
   return QFX_OK;
 }
 

@@ -255,3 +255,19 @@ def test_amax_partials(ctx, n):
     assert parts.max() == np.float32(30.0)
     assert parts.max() == max(np.abs(x.real).max(), np.abs(x.imag).max())
     X.free(); out.free()
+
+
+def test_staged_uploads_keep_the_values_at_call_time(ctx):
+    """qfx_copy_h2d_async: the host array is reused right after each call;
+    20 uploads of 1 MiB also wrap the 16 MiB staging area once."""
+    n = (1 << 20) // 8
+    host = np.zeros(n, np.complex64)
+    bufs = [ctx.alloc(n * 8) for _ in range(20)]
+    for i, b in enumerate(bufs):
+        host[:] = i + 1j * (i + 1)
+        ctx.upload_async(b, host)
+    host[:] = -1
+    for i, b in enumerate(bufs):
+        got = ctx.download(b, np.complex64, (n,))
+        assert np.all(got == np.complex64(i + 1j * (i + 1))), i
+        b.free()
