@@ -317,9 +317,14 @@ void Tensor::reorder(std::vector<std::string> new_ordering, s_type* scratch_copy
                     _string_vector_to_string(_indices), ".");
   }
   if (new_ordering == _indices) return;
-  device_reorder(_data, scratch_copy, _indices, _dimensions, new_ordering);
-  qfx_check(qfx_copy_d2d(device(), _data, scratch_copy, size() * sizeof(s_type)),
-            "qfx_copy_d2d");
+  // The permutation goes into a fresh arena buffer of the same capacity,
+  // which replaces _data (the reference permutes into scratch_copy and
+  // copies back; the copy is saved here).  The old buffer returns to the
+  // arena; work on the stream that reads it is ordered before any reuse.
+  s_type* fresh = device_alloc(_capacity);
+  device_reorder(_data, fresh, _indices, _dimensions, new_ordering);
+  device_free(_data);
+  _data = fresh;
   std::vector<std::size_t> new_dimensions(new_ordering.size());
   for (std::size_t i = 0; i < new_ordering.size(); ++i)
     new_dimensions[i] = _index_to_dimension.at(new_ordering[i]);
