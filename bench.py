@@ -341,13 +341,17 @@ def main():
           if v["relerr_vs_complex128"] is not None and v["relerr_vs_complex128"] <= ERR_BAR}
     pick = max(ok or variants, key=lambda k: variants[k]["amplitudes_per_s"])
     best = variants[pick]
+    # reorders: the variant that moves the most bytes through them (fp32,
+    # which reorders every operand; the split variants read operands in their
+    # stored order and keep only a few small reorders)
+    perm_variant = max(variants, key=lambda k: variants[k]["_prof"]["permute"]["bytes"])
+    p = variants[perm_variant]["_prof"]["permute"]
+    p_ms = max(p["ms"], 1e-9)
     prof = best.pop("_prof")
     for v in variants.values():
         v.pop("_prof", None)
     g = prof["cgemm"]
     achieved = g["flops"] / g["ms"] / 1e9
-    p = prof["permute"]
-    p_ms = max(p["ms"], 1e-9)
     out = {
         "metric": METRIC,
         "value": best["amplitudes_per_s"],
@@ -373,7 +377,8 @@ def main():
                      "algorithmic_bytes_per_launch": g["bytes"] / max(g["launches"], 1),
                      "avg_launch_ms": g["ms"] / max(g["launches"], 1),
                      "flops_per_launch": g["flops"] / max(g["launches"], 1)},
-        "roofline_permute": {"bound": "hbm", "achieved": p["bytes"] / p_ms / 1e6,
+        "roofline_permute": {"bound": "hbm", "variant": perm_variant,
+                             "achieved": p["bytes"] / p_ms / 1e6,
                              "peak": HBM_GBPS, "unit": "GB/s",
                              "frac": p["bytes"] / p_ms / 1e6 / HBM_GBPS,
                              "avg_launch_ms": p["ms"] / max(p["launches"], 1),
