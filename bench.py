@@ -213,7 +213,7 @@ def microbench(args):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cgemm_traffic_7x7_d32.json")
 MFMA_GEMM_KERNELS = ("cgemm_split_big_kernel", "cgemm_fp16x2_mfma_kernel", "cgemm_f32_mfma_kernel",
                      "cgemm_bf16_mfma_kernel")
-GEMM_HELPER_KERNELS = ("amax_kernel", "zero_u32_kernel")
+GEMM_HELPER_KERNELS = ("amax_partial_kernel",)
 
 
 def traffic_for(variant):
