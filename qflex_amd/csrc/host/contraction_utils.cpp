@@ -370,7 +370,7 @@ ContractionData ContractionData::Initialize(const std::list<ContractionOperation
   data.amax_store_.reserve(gemm_outputs.size());
   for (const auto& name : gemm_outputs) {
     const Tensor& buf = data.scratch_[data.scratch_map_.at(name)];
-    const int cap = int(buf.capacity() / 8192 + 64);
+    const int cap = int(buf.capacity() / 2048 + 64);
     data.amax_store_.push_back(Tensor({""}, {std::size_t(cap + 1) / 2}));
     data.amax_slots_[buf.data()] = {reinterpret_cast<float*>(data.amax_store_.back().data()), cap};
   }

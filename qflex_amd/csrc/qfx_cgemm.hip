@@ -1379,11 +1379,23 @@ constexpr int kSmallK = 16;
 // Split-K use (blockIdx.z = chunk > 0 or part != nullptr): the block covers
 // k in [16*z, 16*z + K) of a common dimension of length ldk and writes its
 // partial tile to part[z][m][n] (summed afterwards in fixed order).
+// Max of v over a 256-thread block, written by thread 0 (the partial maxima
+// a split-FP16 consumer of the output reduces instead of a max-abs pass).
+__device__ __forceinline__ void store_block_max(float v, float* slot) {
+  __shared__ float red[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) *slot = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
 template <int BN, bool MAPPED>
 __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
     uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
-    qfx_index_map b_cols, bool a_mfast, bool b_nfast, uint64_t ldk, float2* __restrict__ part) {
+    qfx_index_map b_cols, bool a_mfast, bool b_nfast, uint64_t ldk, float2* __restrict__ part,
+    float* __restrict__ cmax) {
   const uint64_t kbeg = uint64_t(blockIdx.z) * kSmallK;
   constexpr int TX = BN / 4;        // column groups
   constexpr int BM = 4 * 256 / TX;  // rows per block
@@ -1448,6 +1460,7 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
       for (int j = 0; j < 4; ++j) acc[i][j] = cfma(a[i], b[j], acc[i][j]);
   }
   float2* out = part ? part + uint64_t(blockIdx.z) * M * N : C;
+  float cm = 0.f;  // max |re|, |im| of the values this thread stores
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const uint64_t r = m0 + ty * 4 + i;
@@ -1455,9 +1468,13 @@ __global__ __launch_bounds__(256) void cgemm_smallk_kernel(
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int c = n0 + tx + TX * j;
-      if (c < N) out[r * N + c] = acc[i][j];
+      if (c < N) {
+        out[r * N + c] = acc[i][j];
+        cm = fmaxf(cm, fmaxf(fabsf(acc[i][j].x), fabsf(acc[i][j].y)));
+      }
     }
   }
+  if (cmax) store_block_max(cm, cmax + blockIdx.x + uint64_t(gridDim.x) * blockIdx.y);
 }
 
 
@@ -1526,7 +1543,7 @@ template <bool MAPPED>
 __global__ __launch_bounds__(256) void cgemm_rows_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
     uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
-    qfx_index_map b_cols) {
+    qfx_index_map b_cols, float* __restrict__ cmax) {
   __shared__ float2 Bs[kSmallK][16];
   __shared__ uint64_t acol[kSmallK];
   __shared__ float2 Cs[256][17];
@@ -1543,6 +1560,7 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
   __syncthreads();
   const uint64_t m0 = uint64_t(blockIdx.x) * 256;
   const uint64_t row = m0 + threadIdx.x;
+  float cm = 0.f;
   if (row < M) {
     const uint64_t abase = MAPPED ? map_offset(a_rows, row) : row * uint64_t(K);
     float2 acc[16];
@@ -1557,13 +1575,17 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
       }
     }
 #pragma unroll
-    for (int j = 0; j < 16; ++j) Cs[threadIdx.x][j] = acc[j];
+    for (int j = 0; j < 16; ++j) {
+      Cs[threadIdx.x][j] = acc[j];
+      if (j < N) cm = fmaxf(cm, fmaxf(fabsf(acc[j].x), fabsf(acc[j].y)));
+    }
   }
   __syncthreads();
   const uint64_t rows = M - m0 < 256 ? M - m0 : 256;
   const int total = int(rows) * N;
   float2* out = C + m0 * uint64_t(N);
   for (int e = threadIdx.x; e < total; e += 256) out[e] = Cs[e / N][e % N];
+  if (cmax) store_block_max(cm, cmax + blockIdx.x);
 }
 
 unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, 65535))); }
@@ -1597,18 +1619,18 @@ static int launch_smallk_splitk(qfx_context* ctx, const float2* A, const float2*
     dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), unsigned(nchunk)};
     if (mapped)
       cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(
-          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, a_mfast, b_nfast, k, part);
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, a_mfast, b_nfast, k, part, nullptr);
     else
       cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(
-          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, false, true, k, part);
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, false, true, k, part, nullptr);
   } else {
     dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), unsigned(nchunk)};
     if (mapped)
       cgemm_smallk_kernel<64, true><<<g, 256, 0, ctx->stream>>>(
-          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, a_mfast, b_nfast, k, part);
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, a_mfast, b_nfast, k, part, nullptr);
     else
       cgemm_smallk_kernel<64, false><<<g, 256, 0, ctx->stream>>>(
-          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, false, true, k, part);
+          A, B, C, m, int(n), kSmallK, ar, ac, br, bc, false, true, k, part, nullptr);
   }
   sum_partials_kernel<<<cap_grid((m * n + 63) / 64), 256, 0, ctx->stream>>>(part, C, nchunk,
                                                                            m * n);
@@ -1733,17 +1755,17 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     if (n <= 16 && (m + 255) / 256 <= 0x7fffffffULL) {
       cgemm_rows_kernel<false><<<unsigned((m + 255) / 256), 256, 0, ctx->stream>>>(
-          A2, B2, C2, m, int(n), int(k), {}, {}, {}, {});
+          A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr);
     } else if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
                                                                  {}, {}, {}, {}, false, true, k,
-                                                                 nullptr);
+                                                                 nullptr, nullptr);
     } else {
       dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
       cgemm_smallk_kernel<64, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
                                                                  {}, {}, {}, {}, false, true, k,
-                                                                 nullptr);
+                                                                 nullptr, nullptr);
     }
     QFX_HIP(hipGetLastError());
     return QFX_OK;
@@ -1909,19 +1931,28 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     const bool a_mfast = contiguous(*a_rows, 1) && !contiguous(*a_cols, 3);
     const bool b_nfast = contiguous(*b_cols, 1) || !contiguous(*b_rows, 1);
+    // per-block maxima of C for a split-FP16 product that reads it next
+    auto cmax_for = [&](uint64_t blocks) -> float* {
+      if (precision != QFX_PREC_SPLIT_FP16 || !io || !io->c_max ||
+          blocks > uint64_t(std::max(io->c_max_cap, 0)))
+        return nullptr;
+      io->c_max_n = int(blocks);
+      return io->c_max;
+    };
     if (n <= 16 && (m + 255) / 256 <= 0x7fffffffULL) {
-      cgemm_rows_kernel<true><<<unsigned((m + 255) / 256), 256, 0, ctx->stream>>>(
-          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols);
+      const uint64_t blocks = (m + 255) / 256;
+      cgemm_rows_kernel<true><<<unsigned(blocks), 256, 0, ctx->stream>>>(
+          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks));
     } else if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(
           A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast, k,
-          nullptr);
+          nullptr, cmax_for(uint64_t(g.x) * g.y));
     } else {
       dim3 g{unsigned((m + 63) / 64), unsigned((n + 63) / 64), 1};
       cgemm_smallk_kernel<64, true><<<g, 256, 0, ctx->stream>>>(
           A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, a_mfast, b_nfast, k,
-          nullptr);
+          nullptr, cmax_for(uint64_t(g.x) * g.y));
     }
     QFX_HIP(hipGetLastError());
     return QFX_OK;

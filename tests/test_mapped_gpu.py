@@ -171,3 +171,44 @@ def test_scale_chain_matches_recomputed_scales(ctx):
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
     for buf in (A, B1, B2, C1, C2, C2ref, CM):
         buf.free()
+
+
+@pytest.mark.parametrize("k1,n1", [(16, 16), (16, 256)])
+def test_scale_chain_from_small_k_products(ctx, k1, n1):
+    # The vector-ALU products (k <= 16: one thread per row for n <= 16, tiles
+    # otherwise) also write per-block maxima of |C| for a split-FP16
+    # consumer; the next product reads C with them: same scale, same bits.
+    rng = np.random.default_rng(13 + n1)
+    m, n = 524288 if n1 == 16 else 32768, 128  # the second product needs >= 128 large tiles
+    a = (rng.standard_normal((m, k1)) + 1j * rng.standard_normal((m, k1))).astype(np.complex64)
+    b1 = (rng.standard_normal((k1, n1)) + 1j * rng.standard_normal((k1, n1))).astype(np.complex64)
+    b2 = (rng.standard_normal((n1, n)) + 1j * rng.standard_normal((n1, n))).astype(np.complex64)
+    A, B1, B2 = ctx.upload(a), ctx.upload(b1), ctx.upload(b2)
+    C1, C2, C2ref = ctx.alloc(m * n1 * 8), ctx.alloc(m * n * 8), ctx.alloc(m * n * 8)
+    cap = 8192
+    CM = ctx.alloc(4 * cap)
+    nat = lambda r, c: (capi.index_map([r, c], [0]), capi.index_map([r, c], [1]))
+    io1 = capi.ScaleIO(None, 0, None, 0, CM.ptr, cap, 0)
+    assert ctx.cgemm_mapped_ex(capi.SPLIT_FP16, A, *nat(m, k1), B1, *nat(k1, n1), C1, io1)
+    assert 0 < io1.c_max_n <= cap
+    cmax = ctx.download(CM, np.float32, (io1.c_max_n,))
+    c1 = ctx.download(C1, np.complex64, (m, n1))
+    assert cmax.max() == max(np.abs(c1.real).max(), np.abs(c1.imag).max())
+    # C1 = A B1 exactly as the defined-order kernels compute it
+    ref1 = a.astype(np.complex128) @ b1.astype(np.complex128)
+    assert np.linalg.norm(c1 - ref1) / np.linalg.norm(ref1) < 2e-6
+    io2 = capi.ScaleIO(CM.ptr, io1.c_max_n, None, 0, None, 0, 0)
+    if n1 == 16:
+        # m x 16 read as (m / 16) x 256: the next product's k covers whole rows of C1
+        mm, kk = m // 16, 256
+    else:
+        mm, kk = m, n1
+    b2k = np.resize(b2, (kk, n)).astype(np.complex64) if kk != n1 else b2
+    B2k = ctx.upload(b2k)
+    assert ctx.cgemm_mapped_ex(capi.SPLIT_FP16, C1, *nat(mm, kk), B2k, *nat(kk, n), C2, io2)
+    assert ctx.cgemm_mapped(capi.SPLIT_FP16, C1, *nat(mm, kk), B2k, *nat(kk, n), C2ref)
+    got = ctx.download(C2, np.complex64, (mm, n))
+    ref = ctx.download(C2ref, np.complex64, (mm, n))
+    assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
+    for buf in (A, B1, B2, B2k, C1, C2, C2ref, CM):
+        buf.free()
