@@ -73,7 +73,7 @@ void Tensor::_copy(const Tensor& other) {
               "qfx_copy_d2d");
 }
 
-void Tensor::_move(Tensor&& other) {
+void Tensor::_move(Tensor&& other) noexcept {
   _clear();
   _indices = std::move(other._indices);
   _dimensions = std::move(other._dimensions);
@@ -118,7 +118,7 @@ Tensor::Tensor(const std::vector<std::string>& indices,
 
 Tensor::Tensor(const Tensor& other) { _copy(other); }
 
-Tensor::Tensor(Tensor&& other) { _move(std::move(other)); }
+Tensor::Tensor(Tensor&& other) noexcept { _move(std::move(other)); }
 
 Tensor::~Tensor() { _clear(); }
 

@@ -29,7 +29,9 @@ class Tensor {
   Tensor(const std::vector<std::string>& indices, const std::vector<std::size_t>& dimensions,
          const std::vector<s_type>& data);
   Tensor(const Tensor& other);
-  Tensor(Tensor&& other);
+  // noexcept, so that std::vector<Tensor> moves (instead of copying device
+  // buffers) when it grows.
+  Tensor(Tensor&& other) noexcept;
   ~Tensor();
 
   // Copies into existing capacity when this tensor has storage (throws if
@@ -76,7 +78,7 @@ class Tensor {
   void _init(const std::vector<std::string>& indices, const std::vector<std::size_t>& dimensions);
   void _clear();
   void _copy(const Tensor& other);
-  void _move(Tensor&& other);
+  void _move(Tensor&& other) noexcept;
 };
 
 // C = A . B over the shared indices; A and B are reordered in place to
