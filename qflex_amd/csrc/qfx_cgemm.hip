@@ -1317,7 +1317,7 @@ __global__ __launch_bounds__(256) void cgemm_splitk_partial_kernel(
 // output o = i*8 + j on lane o of each wave; the 4 waves are summed in LDS
 // in fixed order and sum_partials_kernel adds the blocks' partials in
 // fixed order, so results are deterministic.
-constexpr int kTinyChunk = 1024;  // k values per block
+constexpr int kTinyChunk = 2048;  // k values per block (a power of two >= 256)
 __global__ __launch_bounds__(256) void cgemm_tiny_mn_partial_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ part,
     int M, int N, uint64_t K) {
@@ -1495,8 +1495,13 @@ __global__ __launch_bounds__(256) void cgemm_tiny_mn_mapped_partial_kernel(
   for (int o = 0; o < 64; ++o) acc[o] = make_float2(0.f, 0.f);
   const uint64_t k0 = uint64_t(blockIdx.x) * kTinyChunk;
   const uint64_t k1 = k0 + kTinyChunk < K ? k0 + kTinyChunk : K;
+  // A digit map moves each index bit to its own offset bit, so indices with
+  // disjoint bits have additive offsets: k0 (a multiple of kTinyChunk =
+  // 2^11), the thread (< 2^8) and the step 256 it (< 2^11) are such.
+  const uint64_t ak0 = map_offset(a_cols, k0 + threadIdx.x), bk0 = map_offset(b_rows, k0 + threadIdx.x);
   for (uint64_t k = k0 + threadIdx.x; k < k1; k += 256) {
-    const uint64_t ak = map_offset(a_cols, k), bk = map_offset(b_rows, k);
+    const uint64_t st = k - k0 - threadIdx.x;
+    const uint64_t ak = ak0 + map_offset(a_cols, st), bk = bk0 + map_offset(b_rows, st);
     float2 a[8], b[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) a[i] = i < M ? A[ar[i] + ak] : make_float2(0.f, 0.f);
