@@ -1544,14 +1544,23 @@ __global__ __launch_bounds__(256) void cgemm_tiny_mn_mapped_partial_kernel(
 // the block's rows of C (one contiguous range of 256 n elements) go through
 // LDS so that the stores are contiguous across the wave.  Each output is the
 // same fmaf chain in increasing k as in cgemm_smallk_kernel (bit-identical).
-template <bool MAPPED>
+// AROW: each row of A holds its K (even) values contiguously and 16-byte
+// aligned; the block's rows are then read into LDS as 16-byte pieces, K / 2
+// consecutive lanes per row, instead of one 8-byte load per lane and k
+// strided by the row pitch.
+template <bool MAPPED, bool AROW>
 __global__ __launch_bounds__(256) void cgemm_rows_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
     uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
     qfx_index_map b_cols, float* __restrict__ cmax) {
+  constexpr int kAPitch = 18;  // 144-byte rows: 16-byte reads of 8 or 16 rows hit distinct banks
+  constexpr int kCPitch = 17;
   __shared__ float2 Bs[kSmallK][16];
   __shared__ uint64_t acol[kSmallK];
-  __shared__ float2 Cs[256][17];
+  // The block's rows of A on their way in (AROW), then its rows of C on
+  // their way out; a barrier separates the two uses.
+  __shared__ __align__(16) float2 X[256 * kAPitch];
+  const uint64_t m0 = uint64_t(blockIdx.x) * 256;
   {
     const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
     float2 v = make_float2(0.f, 0.f);
@@ -1562,26 +1571,55 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
       acol[threadIdx.x] = MAPPED && int(threadIdx.x) < K ? map_offset(a_cols, threadIdx.x)
                                                           : threadIdx.x;
   }
+  if constexpr (AROW) {
+    const int per_row = K >> 1;  // 16-byte pieces per row
+    for (int q = threadIdx.x; q < 256 * per_row; q += 256) {
+      const int r = q / per_row, p = q - r * per_row;
+      const uint64_t grow = m0 + r;
+      float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (grow < M) {
+        const uint64_t off = MAPPED ? map_offset(a_rows, grow) : grow * uint64_t(K);
+        v = *reinterpret_cast<const float4*>(A + off + 2 * p);
+      }
+      *reinterpret_cast<float4*>(&X[r * kAPitch + 2 * p]) = v;
+    }
+  }
   __syncthreads();
-  const uint64_t m0 = uint64_t(blockIdx.x) * 256;
   const uint64_t row = m0 + threadIdx.x;
-  float cm = 0.f;
+  float2 acc[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) acc[j] = make_float2(0.f, 0.f);
   if (row < M) {
-    const uint64_t abase = MAPPED ? map_offset(a_rows, row) : row * uint64_t(K);
-    float2 acc[16];
+    if constexpr (AROW) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = make_float2(0.f, 0.f);
+      for (int kk = 0; kk < kSmallK; kk += 2) {
+        if (kk < K) {
+          const float4 a2 = *reinterpret_cast<const float4*>(&X[threadIdx.x * kAPitch + kk]);
+          const float2 a0 = make_float2(a2.x, a2.y), a1 = make_float2(a2.z, a2.w);
 #pragma unroll
-    for (int kk = 0; kk < kSmallK; ++kk) {
-      if (kk < K) {
-        const float2 a = A[abase + acol[kk]];
+          for (int j = 0; j < 16; ++j) acc[j] = cfma(a0, Bs[kk][j], acc[j]);
 #pragma unroll
-        for (int j = 0; j < 16; ++j) acc[j] = cfma(a, Bs[kk][j], acc[j]);
+          for (int j = 0; j < 16; ++j) acc[j] = cfma(a1, Bs[kk + 1][j], acc[j]);
+        }
+      }
+    } else {
+      const uint64_t abase = MAPPED ? map_offset(a_rows, row) : row * uint64_t(K);
+#pragma unroll
+      for (int kk = 0; kk < kSmallK; ++kk) {
+        if (kk < K) {
+          const float2 a = A[abase + acol[kk]];
+#pragma unroll
+          for (int j = 0; j < 16; ++j) acc[j] = cfma(a, Bs[kk][j], acc[j]);
+        }
       }
     }
+  }
+  if constexpr (AROW) __syncthreads();  // every row of A is read before X takes C
+  float cm = 0.f;
+  if (row < M) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-      Cs[threadIdx.x][j] = acc[j];
+      X[threadIdx.x * kCPitch + j] = acc[j];
       if (j < N) cm = fmaxf(cm, fmaxf(fabsf(acc[j].x), fabsf(acc[j].y)));
     }
   }
@@ -1589,7 +1627,7 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
   const uint64_t rows = M - m0 < 256 ? M - m0 : 256;
   const int total = int(rows) * N;
   float2* out = C + m0 * uint64_t(N);
-  for (int e = threadIdx.x; e < total; e += 256) out[e] = Cs[e / N][e % N];
+  for (int e = threadIdx.x; e < total; e += 256) out[e] = X[(e / N) * kCPitch + e % N];
   if (cmax) store_block_max(cm, cmax + blockIdx.x);
 }
 
@@ -1759,8 +1797,13 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
     // precision (bit-identical to the defined-order scalar kernel).
     QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
     if (n <= 16 && (m + 255) / 256 <= 0x7fffffffULL) {
-      cgemm_rows_kernel<false><<<unsigned((m + 255) / 256), 256, 0, ctx->stream>>>(
-          A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr);
+      const unsigned blocks = unsigned((m + 255) / 256);
+      if (k % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0)
+        cgemm_rows_kernel<false, true><<<blocks, 256, 0, ctx->stream>>>(
+            A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr);
+      else
+        cgemm_rows_kernel<false, false><<<blocks, 256, 0, ctx->stream>>>(
+            A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr);
     } else if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, false><<<g, 256, 0, ctx->stream>>>(A2, B2, C2, m, int(n), int(k),
@@ -1895,6 +1938,24 @@ static bool map_size(const qfx_index_map* d, uint64_t* size) {
   return true;
 }
 
+// The map is the identity: its digits in order from stride 1, index i at
+// offset i.
+static bool identity_map(const qfx_index_map& d) {
+  int at = 0;
+  for (int j = 0; j < d.ndigits; ++j) {
+    if (d.shift[j] != at) return false;
+    at += d.bits[j];
+  }
+  return true;
+}
+
+// Some digit of the map sits at stride 2^shift.
+static bool has_digit_at(const qfx_index_map& d, int shift) {
+  for (int j = 0; j < d.ndigits; ++j)
+    if (d.shift[j] == shift) return true;
+  return false;
+}
+
 // Digit 0 at stride 1 with at least `bits` bits: runs of 2^bits contiguous
 // elements along this matrix dimension.
 static bool contiguous(const qfx_index_map& d, int bits) {
@@ -1946,8 +2007,15 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
     };
     if (n <= 16 && (m + 255) / 256 <= 0x7fffffffULL) {
       const uint64_t blocks = (m + 255) / 256;
-      cgemm_rows_kernel<true><<<unsigned(blocks), 256, 0, ctx->stream>>>(
-          A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks));
+      // rows of A contiguous along k (A's k digits in order from stride 1)
+      // and starting at even offsets (16-byte aligned)
+      if (k % 2 == 0 && identity_map(*a_cols) && !has_digit_at(*a_rows, 0) &&
+          reinterpret_cast<uintptr_t>(A) % 16 == 0)
+        cgemm_rows_kernel<true, true><<<unsigned(blocks), 256, 0, ctx->stream>>>(
+            A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks));
+      else
+        cgemm_rows_kernel<true, false><<<unsigned(blocks), 256, 0, ctx->stream>>>(
+            A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks));
     } else if (n <= 16) {
       dim3 g{unsigned((m + 255) / 256), unsigned((n + 15) / 16), 1};
       cgemm_smallk_kernel<16, true><<<g, 256, 0, ctx->stream>>>(

@@ -134,7 +134,7 @@ def test_cgemm_reference_known_answer(ctx):
         assert np.array_equal(got.ravel(), np.array([20, 14, 100, 78], dtype=np.complex64))
 
 
-@pytest.mark.parametrize("m,n,k", [(70000, 16, 16), (3000, 7, 11), (1000, 100, 16)])
+@pytest.mark.parametrize("m,n,k", [(70000, 16, 16), (3000, 7, 11), (1000, 100, 16), (5000, 9, 6)])
 def test_small_k_kernels_bit_identical_to_defined_order(ctx, m, n, k):
     """k <= 16 runs on the vector-ALU kernels (one thread per row for n <= 16,
     tiles otherwise); appending a zero column to A and a zero row to B makes

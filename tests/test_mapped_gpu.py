@@ -212,3 +212,43 @@ def test_scale_chain_from_small_k_products(ctx, k1, n1):
     assert np.array_equal(got.view(np.uint64), ref.view(np.uint64))
     for buf in (A, B1, B2, B2k, C1, C2, C2ref, CM):
         buf.free()
+
+
+# k <= 16, n <= 16: one thread per row of C.  With A's k digits stored
+# innermost and in order, the rows of A are read whole into LDS.
+ROWS_CASES = [
+    ([16, 16, 16, 16], [16], [16], "common", "right"),   # 2^16 x 16 x 16, A along k
+    ([16, 16, 8], [2, 8], [8], "common", "common"),      # k in two digits, B along k
+    ([16, 16, 16, 4], [16], [16], "left", "right"),      # A along m
+]
+
+
+@pytest.mark.parametrize("case", range(len(ROWS_CASES)))
+def test_mapped_rows_bit_identical_to_defined_order(ctx, case):
+    """Same fmaf chain per output as the scalar kernel: the reordered
+    matrices with a zero column / row appended (k + 1 goes to the scalar
+    kernel in FP32_EXACT) give the same bits."""
+    a_dims, a_left, a_common, b_dims, b_common, b_right = make_case(*ROWS_CASES[case],
+                                                                    seed=40 + case)
+    rng = np.random.default_rng(40 + case)
+    a = (rng.standard_normal(a_dims) + 1j * rng.standard_normal(a_dims)).astype(np.complex64)
+    b = (rng.standard_normal(b_dims) + 1j * rng.standard_normal(b_dims)).astype(np.complex64)
+    m = int(np.prod([a_dims[x] for x in a_left]))
+    k = int(np.prod([a_dims[x] for x in a_common]))
+    n = int(np.prod([b_dims[x] for x in b_right]))
+    am = a.transpose(a_left + a_common).reshape(m, k)
+    bm = b.transpose(b_common + b_right).reshape(k, n)
+    maps = (capi.index_map(a_dims, a_left), capi.index_map(a_dims, a_common),
+            capi.index_map(b_dims, b_common), capi.index_map(b_dims, b_right))
+    A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+    am1 = np.ascontiguousarray(np.concatenate([am, np.zeros((m, 1), np.complex64)], axis=1))
+    bm1 = np.ascontiguousarray(np.concatenate([bm, np.zeros((1, n), np.complex64)], axis=0))
+    A1, B1, C1 = ctx.upload(am1), ctx.upload(bm1), ctx.alloc(m * n * 8)
+    for prec in (capi.FP32_EXACT, capi.SPLIT_FP16):
+        assert ctx.cgemm_mapped(prec, A, maps[0], maps[1], B, maps[2], maps[3], C)
+        ctx.cgemm(capi.FP32_EXACT, m, n, k + 1, A1, B1, C1)
+        got = ctx.download(C, np.complex64, (m, n))
+        want = ctx.download(C1, np.complex64, (m, n))
+        assert np.array_equal(got, want), (case, prec)
+    for buf in (A, B, C, A1, B1, C1):
+        buf.free()
