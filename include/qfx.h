@@ -97,9 +97,12 @@ int qfx_arena_stats(qfx_context* ctx, size_t* reserved, size_t* in_use);
 int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t bytes);
 int qfx_copy_d2h(qfx_context* ctx, void* dst, const void* src, size_t bytes);
 /* Host-to-device copy that returns at once: the bytes are copied into a
- * pinned staging area of the context (so `src` may be reused right away) and
- * sent on the stream; copies above 4 MiB take the synchronous path.  For the
- * many small gate tensors of a network build (Tensor's data constructor,
+ * pinned staging area of the context (so `src` may be reused right away).
+ * The staged copies are batched and moved by one launch, which every later
+ * call that puts work on the stream, reads device memory or waits for the
+ * stream issues first, so they land in call order before any use of `dst`.
+ * Copies above 4 MiB take the synchronous path.  For the many small gate
+ * tensors of a network build (Tensor's data constructor,
  * src/tensor.cpp:170-188). */
 int qfx_copy_h2d_async(qfx_context* ctx, void* dst, const void* src, size_t bytes);
 /* Replaces std::copy of Tensor::_copy / project (src/tensor.cpp:107-139,

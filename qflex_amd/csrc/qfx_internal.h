@@ -48,8 +48,18 @@ struct qfx_context {
   bool capturing = false;  // between qfx_graph_begin and qfx_graph_end
   // Pinned staging area of qfx_copy_h2d_async: filled from offset
   // stage_off on, rewound only after the stream has been synchronized.
+  // The copies are batched: `uploads` lists the staged copies not yet
+  // launched (destination, offset in the staging area, bytes), `upload_at`
+  // their destination ranges (start -> end); one scatter launch moves them
+  // all before the next call that puts work on the stream.
   unsigned char* stage = nullptr;
+  void* stage_dev = nullptr;  // the staging area's device address
   size_t stage_off = 0;
+  struct Upload {
+    uint64_t dst, src_off, bytes;
+  };
+  std::vector<Upload> uploads;
+  std::map<uintptr_t, uintptr_t> upload_at;
   // Grouped products: host staging of the problem list, the event after its
   // copy to the device, and the compute-unit count (grid of the launch).
   std::vector<unsigned char> group_host;
@@ -70,7 +80,20 @@ int qfx_hip_error(hipError_t e, const char* what);
     if (e_ != hipSuccess) return qfx_hip_error(e_, #call); \
   } while (0)
 
-#define QFX_CHECK_CTX(ctx) \
+// Launches the batched staged uploads (qfx_copy_h2d_async) on the stream.
+int qfx_flush_uploads(qfx_context* ctx);
+
+// Start of every entry point that puts work on the stream, reads device
+// memory from the host or waits for the stream: staged uploads go first.
+#define QFX_CHECK_CTX(ctx)                                                \
+  if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context");      \
+  if (!(ctx)->uploads.empty()) {                                          \
+    const int flush_status_ = qfx_flush_uploads(ctx);                     \
+    if (flush_status_ != QFX_OK) return flush_status_;                    \
+  }
+
+// Entry points that only keep books (arena, profiling records, tables).
+#define QFX_CHECK_CTX_NO_FLUSH(ctx) \
   if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context")
 
 // Bracket a launch with profiling events (no-op unless profiling is on).

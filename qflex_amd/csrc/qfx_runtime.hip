@@ -1,5 +1,6 @@
 // Context, device arena, copies, accumulation and timing of the qfx C-ABI.
 #include <cstring>
+#include <iterator>
 #include <mutex>
 
 #include "qfx_internal.h"
@@ -43,7 +44,7 @@ extern "C" int qfx_context_create(int device, size_t arena_chunk_bytes, qfx_cont
 }
 
 extern "C" int qfx_context_destroy(qfx_context* ctx) {
-  QFX_CHECK_CTX(ctx);
+  QFX_CHECK_CTX_NO_FLUSH(ctx);
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
   for (auto& r : ctx->pending) {
@@ -57,6 +58,7 @@ extern "C" int qfx_context_destroy(qfx_context* ctx) {
   for (auto ev : ctx->tune_ev)
     if (ev) hipEventDestroy(ev);
   if (ctx->group_ev) hipEventDestroy(ctx->group_ev);
+  ctx->uploads.clear();  // their destinations are freed with the arena
   if (ctx->stage) hipHostFree(ctx->stage);
   hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -76,7 +78,7 @@ static size_t round_class(size_t bytes) {
 }
 
 extern "C" int qfx_alloc(qfx_context* ctx, size_t bytes, void** dptr) {
-  QFX_CHECK_CTX(ctx);
+  QFX_CHECK_CTX_NO_FLUSH(ctx);
   if (!dptr) return qfx_set_error(QFX_ERR_INVALID, "null output pointer");
   size_t sz = round_class(bytes);
   auto it = ctx->free_lists.find(sz);
@@ -116,11 +118,20 @@ extern "C" int qfx_alloc(qfx_context* ctx, size_t bytes, void** dptr) {
 }
 
 extern "C" int qfx_free(qfx_context* ctx, void* dptr) {
-  QFX_CHECK_CTX(ctx);
+  QFX_CHECK_CTX_NO_FLUSH(ctx);
   if (!dptr) return QFX_OK;
   auto it = ctx->live.find(dptr);
   if (it == ctx->live.end())
     return qfx_set_error(QFX_ERR_INVALID, "qfx_free: pointer not from this arena");
+  // a batched copy into the block is launched before the block can be
+  // handed out again
+  const uintptr_t b0 = reinterpret_cast<uintptr_t>(dptr), b1 = b0 + it->second;
+  auto next = ctx->upload_at.upper_bound(b0);
+  if ((next != ctx->upload_at.end() && next->first < b1) ||
+      (next != ctx->upload_at.begin() && std::prev(next)->second > b0)) {
+    const int st = qfx_flush_uploads(ctx);
+    if (st != QFX_OK) return st;
+  }
   ctx->free_lists[it->second].push_back(dptr);
   ctx->in_use -= it->second;
   ctx->live.erase(it);
@@ -128,7 +139,7 @@ extern "C" int qfx_free(qfx_context* ctx, void* dptr) {
 }
 
 extern "C" int qfx_arena_stats(qfx_context* ctx, size_t* reserved, size_t* in_use) {
-  QFX_CHECK_CTX(ctx);
+  QFX_CHECK_CTX_NO_FLUSH(ctx);
   size_t r = 0;
   for (auto& c : ctx->chunks) r += c.size;
   if (reserved) *reserved = r;
@@ -282,8 +293,48 @@ extern "C" int qfx_copy_h2d(qfx_context* ctx, void* dst, const void* src, size_t
 
 constexpr size_t kStageBytes = size_t(16) << 20;
 
+// One block per staged copy: the bytes are read from the pinned staging
+// area (host memory the device reads over the bus) and written to the
+// destination.
+__global__ __launch_bounds__(256) void qfx_scatter_uploads_kernel(
+    const unsigned char* __restrict__ stage, const qfx_context::Upload* __restrict__ ups) {
+  const qfx_context::Upload u = ups[blockIdx.x];
+  unsigned char* dst = reinterpret_cast<unsigned char*>(u.dst);
+  const unsigned char* src = stage + u.src_off;
+  if (((u.dst | u.src_off | u.bytes) & 15) == 0) {
+    for (uint64_t i = threadIdx.x; i < u.bytes / 16; i += 256)
+      reinterpret_cast<uint4*>(dst)[i] = reinterpret_cast<const uint4*>(src)[i];
+  } else if (((u.dst | u.src_off | u.bytes) & 7) == 0) {
+    for (uint64_t i = threadIdx.x; i < u.bytes / 8; i += 256)
+      reinterpret_cast<uint2*>(dst)[i] = reinterpret_cast<const uint2*>(src)[i];
+  } else {
+    for (uint64_t i = threadIdx.x; i < u.bytes; i += 256) dst[i] = src[i];
+  }
+}
+
+// The upload records go into the staging area behind the data; both stay
+// untouched until the stream has been synchronized.
+int qfx_flush_uploads(qfx_context* ctx) {
+  if (ctx->uploads.empty()) return QFX_OK;
+  const size_t n = ctx->uploads.size();
+  const size_t off = (ctx->stage_off + 255) & ~size_t(255);
+  std::memcpy(ctx->stage + off, ctx->uploads.data(), n * sizeof(qfx_context::Upload));
+  ctx->stage_off = off + n * sizeof(qfx_context::Upload);
+  ctx->uploads.clear();
+  ctx->upload_at.clear();
+  double bytes = 0;
+  const auto* recs = reinterpret_cast<const qfx_context::Upload*>(ctx->stage + off);
+  for (size_t i = 0; i < n; ++i) bytes += 2.0 * double(recs[i].bytes);
+  QfxLaunchScope scope(ctx, QFX_K_COPY, bytes, 0);
+  const unsigned char* stage_dev = static_cast<const unsigned char*>(ctx->stage_dev);
+  qfx_scatter_uploads_kernel<<<unsigned(n), 256, 0, ctx->stream>>>(
+      stage_dev, reinterpret_cast<const qfx_context::Upload*>(stage_dev + off));
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
 extern "C" int qfx_copy_h2d_async(qfx_context* ctx, void* dst, const void* src, size_t bytes) {
-  QFX_CHECK_CTX(ctx);
+  QFX_CHECK_CTX_NO_FLUSH(ctx);
   QFX_NOT_CAPTURING(ctx, "qfx_copy_h2d_async");
   if (!bytes) return QFX_OK;
   if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_h2d_async: null pointer");
@@ -293,15 +344,31 @@ extern "C" int qfx_copy_h2d_async(qfx_context* ctx, void* dst, const void* src, 
     QFX_HIP(hipHostMalloc(&p, kStageBytes, hipHostMallocDefault));
     ctx->stage = static_cast<unsigned char*>(p);
     ctx->stage_off = 0;
+    QFX_HIP(hipHostGetDevicePointer(&ctx->stage_dev, p, 0));
+  }
+  // A second copy into bytes a batched copy still has to write goes into
+  // the next batch (the copies of one batch run in no particular order).
+  const uintptr_t d0 = reinterpret_cast<uintptr_t>(dst), d1 = d0 + bytes;
+  auto next = ctx->upload_at.upper_bound(d0);
+  const bool overlap = (next != ctx->upload_at.end() && next->first < d1) ||
+                       (next != ctx->upload_at.begin() && std::prev(next)->second > d0);
+  if (overlap) {
+    const int st = qfx_flush_uploads(ctx);
+    if (st != QFX_OK) return st;
   }
   size_t off = (ctx->stage_off + 255) & ~size_t(255);
-  if (off + bytes > kStageBytes) {
+  // room for the data and, behind it, the records of the batch
+  const size_t records = (ctx->uploads.size() + 1) * sizeof(qfx_context::Upload) + 256;
+  if (off + bytes + records > kStageBytes) {
     // every earlier staged copy has to be done before the area is reused
+    const int st = qfx_flush_uploads(ctx);
+    if (st != QFX_OK) return st;
     QFX_HIP(hipStreamSynchronize(ctx->stream));
     off = 0;
   }
   std::memcpy(ctx->stage + off, src, bytes);
-  QFX_HIP(hipMemcpyAsync(dst, ctx->stage + off, bytes, hipMemcpyHostToDevice, ctx->stream));
+  ctx->uploads.push_back({uint64_t(d0), uint64_t(off), uint64_t(bytes)});
+  ctx->upload_at[d0] = d1;
   ctx->stage_off = off + bytes;
   return QFX_OK;
 }

@@ -271,3 +271,74 @@ def test_staged_uploads_keep_the_values_at_call_time(ctx):
         got = ctx.download(b, np.complex64, (n,))
         assert np.all(got == np.complex64(i + 1j * (i + 1))), i
         b.free()
+
+
+def _upload_at(ctx, buf, offset, arr):
+    arr = np.ascontiguousarray(arr)
+    capi.check(capi.load().qfx_copy_h2d_async(ctx.handle, buf.ptr + offset, arr.ctypes.data,
+                                              arr.nbytes))
+
+
+def test_staged_uploads_later_copy_into_pending_bytes_wins(ctx):
+    """Staged copies wait in a batch until the next call that puts work on
+    the stream; a copy into bytes the batch still has to write starts a new
+    batch, so the values of the later call win."""
+    b = ctx.alloc(4096)
+    _upload_at(ctx, b, 0, np.full(512, 1, np.uint64))
+    _upload_at(ctx, b, 1024, np.full(16, 2, np.uint64))
+    _upload_at(ctx, b, 1000, np.full(4, 3, np.uint64))  # overlaps both
+    want = np.full(512, 1, np.uint64)
+    want[128:144] = 2
+    want_b = want.view(np.uint8).copy()
+    want_b[1000:1032] = np.full(4, 3, np.uint64).view(np.uint8)
+    assert np.array_equal(ctx.download(b, np.uint8, (4096,)), want_b)
+    b.free()
+
+
+def test_staged_uploads_every_alignment_in_one_batch(ctx):
+    """16-byte, 8-byte and byte-aligned copies of one batch, each whole."""
+    b16, b8, b1 = ctx.alloc(4096), ctx.alloc(256), ctx.alloc(256)
+    x16 = np.arange(512, dtype=np.uint64) * 3
+    x8 = np.array([7, 8, 9], np.uint64)
+    x1 = np.arange(1, 14, dtype=np.uint8)
+    _upload_at(ctx, b16, 0, x16)
+    _upload_at(ctx, b8, 8, x8)
+    _upload_at(ctx, b1, 3, x1)
+    ctx.zero(b8, 8)  # a launch: the batch goes first
+    assert np.array_equal(ctx.download(b16, np.uint64, (512,)), x16)
+    got8 = ctx.download(b8, np.uint64, (4,))
+    assert got8[0] == 0 and np.array_equal(got8[1:], x8)
+    assert np.array_equal(ctx.download(b1, np.uint8, (16,))[3:16], x1)
+    for b in (b16, b8, b1):
+        b.free()
+
+
+def test_staged_uploads_one_launch_per_batch(ctx):
+    bufs = [ctx.alloc(128) for _ in range(200)]
+    ctx.profile(True)
+    ctx.profile_reset()
+    for i, b in enumerate(bufs):
+        ctx.upload_async(b, np.full(16, i, np.uint64))
+    ctx.sync()
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["copy"]["launches"] == 1 and prof["copy"]["bytes"] == 2.0 * 200 * 128
+    for i, b in enumerate(bufs):
+        assert np.all(ctx.download(b, np.uint64, (16,)) == i), i
+        b.free()
+
+
+def test_staged_upload_into_freed_block_then_reused(ctx):
+    """A block freed with its copy still batched and handed out again: the
+    new owner's values are the ones that stay."""
+    b = ctx.alloc(2048)
+    ptr = b.ptr
+    _upload_at(ctx, b, 0, np.full(256, 5, np.uint64))
+    b.free()
+    c = ctx.alloc(2048)
+    assert c.ptr == ptr  # the arena hands the block out again
+    _upload_at(ctx, c, 0, np.full(256, 6, np.uint64))
+    assert np.all(ctx.download(c, np.uint64, (256,)) == 6)
+    ctx.zero(c)
+    assert np.all(ctx.download(c, np.uint64, (256,)) == 0)
+    c.free()
