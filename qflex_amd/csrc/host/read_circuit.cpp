@@ -310,7 +310,7 @@ void flatten_grid_of_tensors(
       Tensor A = grid_of_tensors[i][j][0];
       for (std::size_t k = 1; k < grid_of_tensors[i][j].size(); ++k) {
         Tensor& B = grid_of_tensors[i][j][k];
-        Tensor C({""}, {result_size(A, B)});
+        Tensor C({""}, {result_size(A, B)}, Tensor::for_overwrite);  // multiply writes all of C
         try {
           multiply(A, B, C, scratch);
         } catch (const std::string& err_msg) {

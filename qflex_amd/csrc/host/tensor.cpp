@@ -99,6 +99,17 @@ Tensor::Tensor(const std::vector<std::string>& indices,
 }
 
 Tensor::Tensor(const std::vector<std::string>& indices,
+               const std::vector<std::size_t>& dimensions, for_overwrite_t) {
+  try {
+    _init(indices, dimensions);
+  } catch (const std::string& err_msg) {
+    throw ERROR_MSG("Failed to call _init(). Error:\n\t[", err_msg, "]");
+  }
+  _capacity = size();
+  _data = device_alloc(_capacity);
+}
+
+Tensor::Tensor(const std::vector<std::string>& indices,
                const std::vector<std::size_t>& dimensions, const std::vector<s_type>& data) {
   try {
     _init(indices, dimensions);

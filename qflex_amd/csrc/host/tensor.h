@@ -25,6 +25,12 @@ class Tensor {
   Tensor();
   // Allocates size() elements of device memory, zero-filled.
   Tensor(const std::vector<std::string>& indices, const std::vector<std::size_t>& dimensions);
+  // Allocates size() elements and leaves them unset: for a result that the
+  // next call overwrites whole (the C of multiply), which then needs no fill.
+  struct for_overwrite_t {};
+  static constexpr for_overwrite_t for_overwrite{};
+  Tensor(const std::vector<std::string>& indices, const std::vector<std::size_t>& dimensions,
+         for_overwrite_t);
   // Allocates and uploads `data` (host values, row-major, first index slowest).
   Tensor(const std::vector<std::string>& indices, const std::vector<std::size_t>& dimensions,
          const std::vector<s_type>& data);
