@@ -84,6 +84,53 @@ def test_reorder_large_rank28_like(ctx):
     dst.free()
 
 
+def test_reorder_2gib_rank28(ctx):
+    """2^28 complex64 (2 GiB, the largest tensors of the 7x7 workloads): a
+    rank-28 binary permutation, checked at sampled output positions."""
+    rng = np.random.default_rng(4)
+    n = 1 << 28
+    x = np.empty(n, np.complex64)
+    x.real = np.arange(n, dtype=np.float32)  # exact below 2^24, distinct enough above
+    x.imag = rng.random(n, dtype=np.float32)
+    dims = [2] * 28
+    perm = list(rng.permutation(28))
+    src = ctx.upload(x)
+    dst = ctx.alloc(x.nbytes)
+    ctx.reorder(src, dst, dims, perm)
+    got = ctx.download(dst, np.complex64, (n,))
+    src.free()
+    dst.free()
+    # output position o has output bits o_j (bit 27 - j is output axis j);
+    # its source position has bit 27 - perm[j] set to o_j
+    outs = rng.integers(0, n, 4096, dtype=np.int64)
+    srcs = np.zeros_like(outs)
+    for j, p in enumerate(perm):
+        srcs |= ((outs >> (27 - j)) & 1) << (27 - p)
+    assert np.array_equal(got[outs], x[srcs])
+
+
+@pytest.mark.parametrize("prec", [capi.FP32, capi.SPLIT_FP16])
+def test_cgemm_operand_above_2gib(ctx, prec):
+    """A of 2.2 GB, past the 32-bit byte offsets of the large-tile kernels:
+    the product runs on the 64-bit indexed kernels; rows sampled against
+    complex128."""
+    m, k, n = (1 << 20) + 5, 264, 20
+    rng = np.random.default_rng(5)
+    a = np.empty((m, k), np.complex64)
+    a.real = rng.random((m, k), dtype=np.float32) - 0.5
+    a.imag = rng.random((m, k), dtype=np.float32) - 0.5
+    b = rand_c64(rng, (k, n))
+    A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+    ctx.cgemm(prec, m, n, k, A, B, C)
+    got = ctx.download(C, np.complex64, (m, n))
+    for buf in (A, B, C):
+        buf.free()
+    rows = np.concatenate([rng.integers(0, m, 2000), np.arange(m - 8, m)])
+    ref = a[rows].astype(np.complex128) @ b.astype(np.complex128)
+    err = np.linalg.norm(got[rows] - ref) / np.linalg.norm(ref)
+    assert err < (2e-6 if prec == capi.FP32 else 5e-6), err
+
+
 GEMM_SHAPES = [
     (2, 2, 2), (4, 2, 2), (16, 16, 16), (64, 16, 8), (100, 50, 37), (128, 64, 16),
     (256, 256, 256), (4096, 256, 256), (1024, 512, 256), (513, 129, 65), (64, 4096, 16),
