@@ -630,30 +630,31 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
 // One output tile (rows m0.., complex columns n0..) of the product; the
 // block's 512 threads all call it.  cmax_slot: where the tile's max |C|
 // goes (or null).
-template <int MODE, int WM, int WN, int J, int AM>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2>
 __device__ __forceinline__ void split_big_tile(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
     int N, int K, AmaxRef amax, const qfx_index_map& a_rows, const qfx_index_map& a_cols,
     const qfx_index_map& b_rows, const qfx_index_map& b_cols, float* __restrict__ cmax_slot,
     const int m0, const int n0) {
-  static_assert(WM * WN == 8, "8 waves");
+  constexpr int NTH = 64 * WM * WN;  // threads of the block
+  static_assert(NTH == 512 || NTH == 256, "8 or 4 waves");
   typedef typename SplitTraits<MODE>::T T;
   typedef typename SplitTraits<MODE>::V V;
   constexpr int NT = SplitTraits<MODE>::NT;
-  constexpr int kBM = 64 * WM, kBN = 16 * J * WN;
+  constexpr int kBM = 32 * I * WM, kBN = 16 * J * WN;
   constexpr int kAPlane = kBM * kRowL;
   constexpr int kBPlane = 2 * kBN * kRowL;
   constexpr int kBuf = NT * (kAPlane + kBPlane);
-  constexpr int kAH = kBM * 4 / 512;            // float4 A loads per thread and stage
+  constexpr int kAH = kBM * 4 / NTH;            // float4 A loads per thread and stage
   constexpr int kBT = 4 * kBN;                  // float4 B loads per stage
-  constexpr int kBH = (kBT + 511) / 512;        // ... per thread
+  constexpr int kBH = (kBT + NTH - 1) / NTH;    // ... per thread
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* smem = reinterpret_cast<T*>(smem_raw);
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wm = wave / WN;  // 64 rows each
+  const int wm = wave / WN;  // 32 I rows each
   const int wn = wave % WN;  // 16*J complex columns each
   float sa = 1.f, sb = 1.f;
 
@@ -666,14 +667,14 @@ __device__ __forceinline__ void split_big_tile(
   if constexpr (AM != 0) {
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
-      const int f = tid + 512 * h;
+      const int f = tid + NTH * h;
       const int row = AM == 1 ? (f >> 2) : 2 * (f >> 3);
       a_off[h] = map_offset(a_rows, uint64_t(m0 + row));
     }
     a_coff = map_offset(a_cols, AM == 1 ? uint64_t(2 * (tid & 3)) : uint64_t(tid & 7));
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
-      const int g = tid + 512 * h;
+      const int g = tid + NTH * h;
       b_off[h] = map_offset(b_cols, uint64_t(n0 + 2 * (g >> 3)));
     }
     b_roff = map_offset(b_rows, uint64_t(tid & 7));
@@ -683,7 +684,7 @@ __device__ __forceinline__ void split_big_tile(
   uint64_t* ktab = reinterpret_cast<uint64_t*>(smem_raw + 2 * kBuf * sizeof(T));
   const int nkt = (K + kBBK - 1) / kBBK;
   if constexpr (AM != 0) {
-    for (int t = tid; t < nkt; t += 512) {
+    for (int t = tid; t < nkt; t += NTH) {
       ktab[2 * t] = map_offset(a_cols, uint64_t(t) * kBBK);
       ktab[2 * t + 1] = map_offset(b_rows, uint64_t(t) * kBBK);
     }
@@ -712,7 +713,7 @@ __device__ __forceinline__ void split_big_tile(
     }
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
-      const int f = tid + 512 * h;
+      const int f = tid + NTH * h;
       int gm, gk;
       unsigned off;
       if constexpr (AM == 2) {
@@ -729,11 +730,11 @@ __device__ __forceinline__ void split_big_tile(
     }
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
-      const int g = tid + 512 * h;
+      const int g = tid + NTH * h;
       const int gk = k0 + (g & 7), gn = n0 + 2 * (g >> 3);
       const unsigned off = AM == 0 ? unsigned(gk) * unsigned(N) + unsigned(gn)
                                    : b_kbase + unsigned(b_roff) + unsigned(b_off[h]);
-      const bool in = (kBT % 512 == 0 || g < kBT) && gk < K && gn < N;
+      const bool in = (kBT % NTH == 0 || g < kBT) && gk < K && gn < N;
       rb[h] = load16(rsrc_b, in ? off * unsigned(sizeof(float2)) : kOutside);
     }
   };
@@ -743,7 +744,7 @@ __device__ __forceinline__ void split_big_tile(
     T* base = smem + buf * kBuf;
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
-      const int f = tid + 512 * h;
+      const int f = tid + NTH * h;
       float v[4] = {ra[h].x * sa, ra[h].y * sa, ra[h].z * sa, ra[h].w * sa};
       T t3[4][3];
 #pragma unroll
@@ -770,8 +771,8 @@ __device__ __forceinline__ void split_big_tile(
     T* bb = base + NT * kAPlane;
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
-      const int g = tid + 512 * h;
-      if (kBT % 512 != 0 && g >= kBT) continue;
+      const int g = tid + NTH * h;
+      if (kBT % NTH != 0 && g >= kBT) continue;
       const int kk = g & 7, nq = g >> 3;
       // (Br, Bi) of n, then of n+1
       float v[4] = {rb[h].x * sb, rb[h].y * sb, rb[h].z * sb, rb[h].w * sb};
@@ -797,9 +798,9 @@ __device__ __forceinline__ void split_big_tile(
   const int col = lane & 31;
   const int part = col & 1;
 
-  floatx16 acc[2][J];
+  floatx16 acc[I][J];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < I; ++i)
 #pragma unroll
     for (int j = 0; j < J; ++j)
 #pragma unroll
@@ -808,13 +809,13 @@ __device__ __forceinline__ void split_big_tile(
   auto compute = [&](int buf) {
     const T* base = smem + buf * kBuf;
     const T* bb = base + NT * kAPlane;
-    V a[NT][2], b[NT][J];
+    V a[NT][I], b[NT][J];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < I; ++i)
         a[t][i] = *reinterpret_cast<const V*>(base + t * kAPlane +
-                                              (wm * 64 + i * 32 + col) * kRowL + 8 * half);
+                                              (wm * 32 * I + i * 32 + col) * kRowL + 8 * half);
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         const int rc = wn * 32 * J + j * 32 + col;  // real column of Bhat in the block
@@ -822,7 +823,7 @@ __device__ __forceinline__ void split_big_tile(
       }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < I; ++i)
 #pragma unroll
       for (int j = 0; j < J; ++j) {
         if constexpr (SplitTraits<MODE>::PRODUCTS == 6) {
@@ -838,8 +839,8 @@ __device__ __forceinline__ void split_big_tile(
   // Interleave of one stage: global loads, fragment reads, then MFMAs in
   // pairs around groups of VALU (operand split) and one LDS store.
   auto schedule = [&]() {
-    constexpr int kMfma = 2 * J * SplitTraits<MODE>::PRODUCTS;
-    __builtin_amdgcn_sched_group_barrier(0x100, (2 + J) * NT, 0);  // DS read
+    constexpr int kMfma = I * J * SplitTraits<MODE>::PRODUCTS;
+    __builtin_amdgcn_sched_group_barrier(0x100, (I + J) * NT, 0);  // DS read
 #pragma unroll
     for (int i = 0; i < kMfma / 2; ++i) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
@@ -857,7 +858,7 @@ __device__ __forceinline__ void split_big_tile(
   // the scales are reduced while the first loads are in flight (in the
   // second stage buffer, first written after the prologue's barrier)
   if constexpr (MODE == 2)
-    operand_scales<512>(amax, reinterpret_cast<float*>(smem + kBuf), sa, sb);
+    operand_scales<NTH>(amax, reinterpret_cast<float*>(smem + kBuf), sa, sb);
   load_tile(min(1, nkt - 1), ra[1], rb[1]);
   store_tile(0, ra[0], rb[0]);
   __syncthreads();
@@ -886,12 +887,12 @@ __device__ __forceinline__ void split_big_tile(
   float* Cf = reinterpret_cast<float*>(C);
   float cm = 0.f;  // max |re|, |im| of the values this thread stores
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < I; ++i)
 #pragma unroll
     for (int j = 0; j < J; ++j)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
+        const int row = m0 + wm * 32 * I + i * 32 + 8 * (r >> 2) + 4 * half + (r & 3);
         const int cn = n0 + wn * 16 * J + j * 16 + (col >> 1);
         if (row < M && cn < N) {
           const float v = acc[i][j][r] * unscale;
@@ -910,21 +911,21 @@ __device__ __forceinline__ void split_big_tile(
     if (tid == 0) {
       float b = red[0];
 #pragma unroll
-      for (int w = 1; w < 8; ++w) b = fmaxf(b, red[w]);
+      for (int w = 1; w < NTH / 64; ++w) b = fmaxf(b, red[w]);
       *cmax_slot = b;
     }
   }
 }
 
-template <int MODE, int WM, int WN, int J, int AM>
-__global__ __launch_bounds__(512) void cgemm_split_big_kernel(
+template <int MODE, int WM, int WN, int J, int AM, int I = 2>
+__global__ __launch_bounds__(64 * WM * WN) void cgemm_split_big_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
     int N, int K, AmaxRef amax, qfx_index_map a_rows,
     qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols,
     float* __restrict__ cmax) {
-  split_big_tile<MODE, WM, WN, J, AM>(
+  split_big_tile<MODE, WM, WN, J, AM, I>(
       A, B, C, M, N, K, amax, a_rows, a_cols, b_rows, b_cols,
-      cmax ? cmax + blockIdx.x + gridDim.x * blockIdx.y : nullptr, int(blockIdx.x) * 64 * WM,
+      cmax ? cmax + blockIdx.x + gridDim.x * blockIdx.y : nullptr, int(blockIdx.x) * 32 * I * WM,
       int(blockIdx.y) * 16 * J * WN);
 }
 
@@ -961,41 +962,41 @@ __global__ __launch_bounds__(512) void cgemm_split_grouped_kernel(const GroupTil
   }
 }
 
-template <int MODE, int WM, int WN, int J>
+template <int MODE, int WM, int WN, int J, int I = 2>
 constexpr size_t split_big_smem() {
-  return size_t(2) * SplitTraits<MODE>::NT * (64 * WM * kRowL + 2 * 16 * J * WN * kRowL) * 2;
+  return size_t(2) * SplitTraits<MODE>::NT * (32 * I * WM * kRowL + 2 * 16 * J * WN * kRowL) * 2;
 }
 
 constexpr size_t kMaxLds = 160 * 1024;
 
 // LDS of one launch: the two stage buffers, plus 16 bytes per stage for the
 // per-stage k offsets of the mapped forms.
-template <int MODE, int WM, int WN, int J, int AM>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2>
 size_t split_big_lds(uint64_t k) {
-  return split_big_smem<MODE, WM, WN, J>() + (AM != 0 ? 16 * ((k + kBBK - 1) / kBBK) : 0);
+  return split_big_smem<MODE, WM, WN, J, I>() + (AM != 0 ? 16 * ((k + kBBK - 1) / kBBK) : 0);
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel once per process.
-template <int MODE, int WM, int WN, int J, int AM>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2>
 hipError_t split_big_prepare() {
   static hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN, J, AM>),
+      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN, J, AM, I>),
       hipFuncAttributeMaxDynamicSharedMemorySize,
-      int(AM != 0 ? kMaxLds : split_big_smem<MODE, WM, WN, J>()));
+      int(AM != 0 ? kMaxLds : split_big_smem<MODE, WM, WN, J, I>()));
   return e;
 }
 
-template <int MODE, int WM, int WN, int J, int AM>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2>
 hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
                             uint64_t m, uint64_t n, uint64_t k, AmaxRef amax,
                             const qfx_index_map* maps, float* cmax) {
-  hipError_t e = split_big_prepare<MODE, WM, WN, J, AM>();
+  hipError_t e = split_big_prepare<MODE, WM, WN, J, AM, I>();
   if (e != hipSuccess) return e;
-  constexpr int kBM = 64 * WM, kBN = 16 * J * WN;
+  constexpr int kBM = 32 * I * WM, kBN = 16 * J * WN;
   dim3 g{unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN), 1};
   qfx_index_map none{};
-  cgemm_split_big_kernel<MODE, WM, WN, J, AM>
-      <<<g, 512, split_big_lds<MODE, WM, WN, J, AM>(k), stream>>>(
+  cgemm_split_big_kernel<MODE, WM, WN, J, AM, I>
+      <<<g, 64 * WM * WN, split_big_lds<MODE, WM, WN, J, AM, I>(k), stream>>>(
           A, B, C, int(m), int(n), int(k), amax, maps ? maps[0] : none, maps ? maps[1] : none,
           maps ? maps[2] : none, maps ? maps[3] : none, cmax);
   return hipGetLastError();
