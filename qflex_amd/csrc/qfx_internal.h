@@ -86,15 +86,19 @@ int qfx_flush_uploads(qfx_context* ctx);
 // Start of every entry point that puts work on the stream, reads device
 // memory from the host or waits for the stream: staged uploads go first.
 #define QFX_CHECK_CTX(ctx)                                                \
-  if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context");      \
-  if (!(ctx)->uploads.empty()) {                                          \
-    const int flush_status_ = qfx_flush_uploads(ctx);                     \
-    if (flush_status_ != QFX_OK) return flush_status_;                    \
-  }
+  do {                                                                    \
+    if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context");    \
+    if (!(ctx)->uploads.empty()) {                                        \
+      const int flush_status_ = qfx_flush_uploads(ctx);                   \
+      if (flush_status_ != QFX_OK) return flush_status_;                  \
+    }                                                                     \
+  } while (0)
 
 // Entry points that only keep books (arena, profiling records, tables).
-#define QFX_CHECK_CTX_NO_FLUSH(ctx) \
-  if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context")
+#define QFX_CHECK_CTX_NO_FLUSH(ctx)                                       \
+  do {                                                                    \
+    if (!(ctx)) return qfx_set_error(QFX_ERR_INVALID, "null context");    \
+  } while (0)
 
 // Bracket a launch with profiling events (no-op unless profiling is on).
 struct QfxLaunchScope {
