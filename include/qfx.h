@@ -142,8 +142,11 @@ int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n, uint6
  * such products (numbered product by product), so many small or uneven
  * products fill the GPU together; each result is bit-identical to that
  * product computed alone on the 256 x 128 tile (split FP16: one max-abs
- * launch per product precedes the grouped launch).  All other products run
- * one by one through qfx_cgemm_c64.  The C_i must not overlap each other or
+ * launch per product precedes the grouped launch).  Products with k <= 16
+ * and n <= 16 (any precision) also share one launch, of the exact-FP32 row
+ * kernel qfx_cgemm_c64 runs them on (same bits); the problem list reaches
+ * the device through the staged uploads.  All other products run one by
+ * one through qfx_cgemm_c64.  The C_i must not overlap each other or
  * any A_j, B_j.  Not allowed while recording a graph.  No counterpart in the
  * reference (one cblas_cgemm per pairwise contraction, src/tensor.cpp:973). */
 typedef struct qfx_gemm_desc {

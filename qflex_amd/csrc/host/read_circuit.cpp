@@ -2,11 +2,14 @@
 #include "read_circuit.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <unordered_map>
 #include <unordered_set>
 
+#include "device.h"
 #include "errors.h"
+#include "global.h"
 
 namespace qflex {
 
@@ -301,26 +304,58 @@ void flatten_grid_of_tensors(
     const std::list<ContractionOperation>& ordering, s_type* scratch) {
   if (scratch == nullptr) throw ERROR_MSG("Scratch must be non-null.");
   const std::size_t I = grid_of_tensors.size();
-  // 1. Contract each qubit's time line into one site tensor.
+  // 1. Contract each qubit's time line into one site tensor, one gate layer
+  //    at a time over all sites (the reference runs each site's line to the
+  //    end; the products are the same): the matrix products of a layer go
+  //    out as one grouped launch, products with a vector operand one by one.
+  std::vector<std::array<std::size_t, 2>> sites;
+  std::vector<Tensor> line_product;
+  std::size_t depth = 0;
   for (std::size_t i = 0; i < I; ++i) {
     const std::size_t J = grid_of_tensors[i].size();
     for (std::size_t j = 0; j < J; ++j) {
       if (find_grid_coord_in_list(off, i, j)) continue;
       if (grid_of_tensors[i][j].empty()) throw ERROR_MSG("Time-direction cannot be empty");
-      Tensor A = grid_of_tensors[i][j][0];
-      for (std::size_t k = 1; k < grid_of_tensors[i][j].size(); ++k) {
-        Tensor& B = grid_of_tensors[i][j][k];
-        Tensor C({""}, {result_size(A, B)}, Tensor::for_overwrite);  // multiply writes all of C
-        try {
-          multiply(A, B, C, scratch);
-        } catch (const std::string& err_msg) {
-          throw ERROR_MSG("Failed to call multiply(). Error:\n\t[", err_msg, "]");
-        }
-        A = std::move(C);
-      }
-      grid_of_tensors_2D[i][j] = std::move(A);
+      sites.push_back({i, j});
+      line_product.push_back(grid_of_tensors[i][j][0]);
+      depth = std::max(depth, grid_of_tensors[i][j].size());
     }
   }
+  std::vector<Tensor> next(sites.size());
+  std::vector<qfx_gemm_desc> layer;
+  for (std::size_t k = 1; k < depth; ++k) {
+    layer.clear();
+    std::vector<std::size_t> stepped;
+    for (std::size_t s = 0; s < sites.size(); ++s) {
+      auto& line = grid_of_tensors[sites[s][0]][sites[s][1]];
+      if (k >= line.size()) continue;
+      Tensor& A = line_product[s];
+      Tensor& B = line[k];
+      Tensor C({""}, {result_size(A, B)}, Tensor::for_overwrite);  // written whole below
+      try {
+        const ProductShape p = prepare_multiply(A, B, C, scratch);
+        if (p.has_left && p.has_right) {
+          layer.push_back(qfx_gemm_desc{p.left_dim, p.right_dim, p.common_dim,
+                                        reinterpret_cast<const qfx_c64*>(A.data()),
+                                        reinterpret_cast<const qfx_c64*>(B.data()),
+                                        reinterpret_cast<qfx_c64*>(C.data())});
+        } else {
+          device_multiply(A.data(), B.data(), C.data(), p.left_dim, p.right_dim, p.common_dim,
+                          p.has_left, p.has_right);
+        }
+      } catch (const std::string& err_msg) {
+        throw ERROR_MSG("Failed to call multiply(). Error:\n\t[", err_msg, "]");
+      }
+      next[s] = std::move(C);
+      stepped.push_back(s);
+    }
+    if (!layer.empty())
+      qfx_check(qfx_cgemm_grouped_c64(device(), global::precision, int(layer.size()), layer.data()),
+                "qfx_cgemm_grouped_c64");
+    for (std::size_t s : stepped) line_product[s] = std::move(next[s]);
+  }
+  for (std::size_t s = 0; s < sites.size(); ++s)
+    grid_of_tensors_2D[sites[s][0]][sites[s][1]] = std::move(line_product[s]);
   // 2. Bundle the virtual indices shared by neighbouring sites into one bond.
   for (std::size_t i = 0; i < I; ++i) {
     const std::size_t J = grid_of_tensors[i].size();

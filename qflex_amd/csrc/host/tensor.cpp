@@ -403,7 +403,7 @@ void device_multiply(const s_type* A, const s_type* B, s_type* C, std::size_t le
   }
 }
 
-void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
+ProductShape prepare_multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
   if (scratch_copy == nullptr) throw ERROR_MSG("Scratch copy must be non-null.");
   if (A.data() == C.data())
     throw ERROR_MSG("A and C cannot be the same tensor: ", C.tensor_to_string());
@@ -413,9 +413,9 @@ void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
   std::vector<std::string> left = _vector_subtraction(A.get_indices(), B.get_indices());
   std::vector<std::string> right = _vector_subtraction(B.get_indices(), A.get_indices());
   std::vector<std::string> common = _vector_intersection(A.get_indices(), B.get_indices());
-  std::size_t left_dim = 1, right_dim = 1, common_dim = 1;
-  for (const auto& i : left) left_dim *= A.get_index_to_dimension().at(i);
-  for (const auto& i : right) right_dim *= B.get_index_to_dimension().at(i);
+  ProductShape p;
+  for (const auto& i : left) p.left_dim *= A.get_index_to_dimension().at(i);
+  for (const auto& i : right) p.right_dim *= B.get_index_to_dimension().at(i);
   for (std::size_t i = 0; i < common.size(); ++i) {
     std::size_t a_dim = A.get_index_to_dimension().at(common[i]);
     std::size_t b_dim = B.get_index_to_dimension().at(common[i]);
@@ -423,11 +423,11 @@ void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
       throw ERROR_MSG("Common indices must have matching dimensions, but at index: ", i,
                       ", the dimensions are A: ", a_dim, ", B: ", b_dim, ".");
     }
-    common_dim *= a_dim;
+    p.common_dim *= a_dim;
   }
-  if (left_dim * right_dim > C.capacity()) {
+  if (p.left_dim * p.right_dim > C.capacity()) {
     throw ERROR_MSG("C: ", C.capacity(), " doesn't have enough space for the product of A*B: ",
-                    (left_dim * right_dim), ".");
+                    (p.left_dim * p.right_dim), ".");
   }
   try {
     A.reorder(_vector_union(left, common), scratch_copy);
@@ -439,8 +439,8 @@ void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
   } catch (const std::string& err_msg) {
     throw ERROR_MSG("Failed to call reorder(). Error:\n\t[", err_msg, "]");
   }
-  device_multiply(A.data(), B.data(), C.data(), left_dim, right_dim, common_dim, !left.empty(),
-                  !right.empty());
+  p.has_left = !left.empty();
+  p.has_right = !right.empty();
 
   std::vector<std::string> C_indices = _vector_union(left, right);
   std::vector<std::size_t> C_dimensions(C_indices.size());
@@ -455,6 +455,13 @@ void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
     throw ERROR_MSG("Failed to call set_dimensions(). Error:\n\t[", err_msg, "]");
   }
   C.generate_index_to_dimension();
+  return p;
+}
+
+void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy) {
+  const ProductShape p = prepare_multiply(A, B, C, scratch_copy);
+  device_multiply(A.data(), B.data(), C.data(), p.left_dim, p.right_dim, p.common_dim,
+                  p.has_left, p.has_right);
 }
 
 std::size_t result_size(Tensor& A, Tensor& B) {

@@ -92,6 +92,16 @@ class Tensor {
 // scratch_copy: device buffer of at least max(A.size(), B.size()) elements.
 void multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy);
 
+// The host half of multiply: the checks, the index algebra, the reorders of
+// A to (left, common) and B to (common, right), and C's indices and
+// dimensions; the product itself is left to the caller (device_multiply, or
+// a grouped launch of several products).
+struct ProductShape {
+  std::size_t left_dim = 1, right_dim = 1, common_dim = 1;
+  bool has_left = false, has_right = false;
+};
+ProductShape prepare_multiply(Tensor& A, Tensor& B, Tensor& C, s_type* scratch_copy);
+
 std::size_t result_size(Tensor& A, Tensor& B);
 
 void bundle_between(Tensor& A, Tensor& B, std::string bundled_index, s_type* scratch_copy);

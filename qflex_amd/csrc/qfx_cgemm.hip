@@ -1549,11 +1549,16 @@ __global__ __launch_bounds__(256) void cgemm_tiny_mn_mapped_partial_kernel(
 // aligned; the block's rows are then read into LDS as 16-byte pieces, K / 2
 // consecutive lanes per row, instead of one 8-byte load per lane and k
 // strided by the row pitch.
+// The 256 rows of C from row 256 * block on (block = blockIdx.x of a single
+// product, or the block's index inside its product in a grouped launch).
 template <bool MAPPED, bool AROW>
-__global__ __launch_bounds__(256) void cgemm_rows_kernel(
-    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
-    uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
-    qfx_index_map b_cols, float* __restrict__ cmax) {
+__device__ __forceinline__ void rows_tile(const float2* __restrict__ A,
+                                          const float2* __restrict__ B, float2* __restrict__ C,
+                                          uint64_t M, int N, int K, const qfx_index_map& a_rows,
+                                          const qfx_index_map& a_cols,
+                                          const qfx_index_map& b_rows,
+                                          const qfx_index_map& b_cols, float* __restrict__ cmax,
+                                          uint64_t block) {
   constexpr int kAPitch = 18;  // 144-byte rows: 16-byte reads of 8 or 16 rows hit distinct banks
   constexpr int kCPitch = 17;
   __shared__ float2 Bs[kSmallK][16];
@@ -1561,7 +1566,7 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
   // The block's rows of A on their way in (AROW), then its rows of C on
   // their way out; a barrier separates the two uses.
   __shared__ __align__(16) float2 X[256 * kAPitch];
-  const uint64_t m0 = uint64_t(blockIdx.x) * 256;
+  const uint64_t m0 = block * 256;
   {
     const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
     float2 v = make_float2(0.f, 0.f);
@@ -1629,7 +1634,42 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
   const int total = int(rows) * N;
   float2* out = C + m0 * uint64_t(N);
   for (int e = threadIdx.x; e < total; e += 256) out[e] = X[(e / N) * kCPitch + e % N];
-  if (cmax) store_block_max(cm, cmax + blockIdx.x);
+  if (cmax) store_block_max(cm, cmax + block);
+}
+
+template <bool MAPPED, bool AROW>
+__global__ __launch_bounds__(256) void cgemm_rows_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
+    uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
+    qfx_index_map b_cols, float* __restrict__ cmax) {
+  rows_tile<MAPPED, AROW>(A, B, C, M, N, K, a_rows, a_cols, b_rows, b_cols, cmax, blockIdx.x);
+}
+
+// One problem of a grouped launch of the row kernel (row-major operands).
+struct RowsProb {
+  const float2* A;
+  const float2* B;
+  float2* C;
+  uint64_t M;
+  int N, K;
+  uint64_t first_block;  // index of the problem's first block in the launch
+};
+
+// Grouped row kernel: block b runs block b - first_block of the last
+// problem whose first block is <= b.
+__global__ __launch_bounds__(256) void cgemm_rows_grouped_kernel(const RowsProb* __restrict__ probs,
+                                                                 int count) {
+  const uint64_t b = blockIdx.x;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (probs[mid].first_block <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const RowsProb p = probs[lo];
+  const qfx_index_map none{};
+  rows_tile<false, false>(p.A, p.B, p.C, p.M, p.N, p.K, none, none, none, none, nullptr,
+                          b - p.first_block);
 }
 
 unsigned cap_grid(uint64_t g) { return unsigned(std::max<uint64_t>(1, std::min<uint64_t>(g, 65535))); }
@@ -2111,6 +2151,38 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
 
 // ------------------------------------------------------------ grouped products
 
+// The row-kernel problems of a grouped call in one launch.  The problem
+// list goes to the device through the staged uploads (flushed right before
+// the launch) into the workspace.
+static int launch_rows_grouped(qfx_context* ctx, const qfx_gemm_desc* descs,
+                               const std::vector<int>& rows) {
+  std::vector<RowsProb> probs(rows.size());
+  uint64_t blocks = 0;
+  double flops = 0, bytes = 0;
+  for (size_t r = 0; r < rows.size(); ++r) {
+    const qfx_gemm_desc& d = descs[rows[r]];
+    probs[r] = RowsProb{reinterpret_cast<const float2*>(d.A), reinterpret_cast<const float2*>(d.B),
+                        reinterpret_cast<float2*>(d.C), d.m, int(d.n), int(d.k), blocks};
+    blocks += (d.m + 255) / 256;
+    flops += 8.0 * double(d.m) * double(d.n) * double(d.k);
+    bytes += 8.0 * (double(d.m) * d.k + double(d.k) * d.n + double(d.m) * d.n);
+  }
+  if (blocks > 0x7fffffffULL)
+    return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_grouped_c64: too many row blocks");
+  void* ws = nullptr;
+  int st = qfx_workspace(ctx, probs.size() * sizeof(RowsProb), &ws);
+  if (st != QFX_OK) return st;
+  st = qfx_copy_h2d_async(ctx, ws, probs.data(), probs.size() * sizeof(RowsProb));
+  if (st != QFX_OK) return st;
+  st = qfx_flush_uploads(ctx);
+  if (st != QFX_OK) return st;
+  QfxLaunchScope scope(ctx, QFX_K_CGEMM_SMALL, bytes, flops);
+  cgemm_rows_grouped_kernel<<<unsigned(blocks), 256, 0, ctx->stream>>>(
+      static_cast<const RowsProb*>(ws), int(probs.size()));
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
 extern "C" int qfx_cgemm_grouped_c64(qfx_context* ctx, int precision, int count,
                                      const qfx_gemm_desc* descs) {
   QFX_CHECK_CTX(ctx);
@@ -2121,12 +2193,19 @@ extern "C" int qfx_cgemm_grouped_c64(qfx_context* ctx, int precision, int count,
     return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_grouped_c64: unknown precision");
   const bool split = precision == QFX_PREC_SPLIT_FP16 || precision == QFX_PREC_SPLIT_BF16 ||
                      precision == QFX_PREC_SPLIT_BF16X3;
-  // Problems for the large tiles go into one launch; the others (other
-  // precisions, k <= 16, odd sizes, unaligned or too large operands) run one
-  // by one exactly as qfx_cgemm_c64 would run them.
-  std::vector<int> grouped;
+  // Problems for the large tiles go into one launch, and so do those of the
+  // row kernel (k, n <= 16, any precision: exact FP32, the kernel
+  // qfx_cgemm_c64 runs them on); the others (other precisions, odd sizes,
+  // unaligned or too large operands) run one by one exactly as
+  // qfx_cgemm_c64 would run them.
+  std::vector<int> grouped, rows;
   for (int i = 0; i < count; ++i) {
     const qfx_gemm_desc& d = descs[i];
+    if (d.m > 0 && d.n > 0 && d.k > 0 && d.k <= uint64_t(kSmallK) && d.n <= 16 && d.A && d.B &&
+        d.C && (d.m + 255) / 256 <= 0x7fffffffULL) {
+      rows.push_back(i);
+      continue;
+    }
     const bool eligible = split && d.m > 0 && d.n > 0 && d.k > uint64_t(kSmallK) &&
                           d.k % 2 == 0 && d.n % 2 == 0 && d.A && d.B && d.C &&
                           reinterpret_cast<uintptr_t>(d.A) % 16 == 0 &&
@@ -2138,6 +2217,10 @@ extern "C" int qfx_cgemm_grouped_c64(qfx_context* ctx, int precision, int count,
       continue;
     }
     const int st = qfx_cgemm_c64(ctx, precision, d.m, d.n, d.k, d.A, d.B, d.C);
+    if (st != QFX_OK) return st;
+  }
+  if (!rows.empty()) {
+    const int st = launch_rows_grouped(ctx, descs, rows);
     if (st != QFX_OK) return st;
   }
   if (grouped.empty()) return QFX_OK;

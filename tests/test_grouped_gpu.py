@@ -93,3 +93,32 @@ def test_grouped_empty_and_refused_while_recording(ctx):
     finally:
         ctx.graph_end().destroy()
     free(probs)
+
+
+ROW_SHAPES = [(65536, 8, 2), (1, 2, 2), (300, 16, 16), (4097, 2, 2), (777, 7, 11), (256, 16, 1),
+              (20000, 8, 2)]
+
+
+@pytest.mark.parametrize("prec", [capi.SPLIT_FP16, capi.FP32])
+def test_grouped_row_kernel_products_same_bits_as_alone(ctx, prec):
+    """k, n <= 16 (the time-line products of a network build): one grouped
+    launch of the exact-FP32 row kernel, each result bit-identical to the
+    product run alone; a large-tile product in the same call is unaffected."""
+    rng = np.random.default_rng(prec + 60)
+    shapes = ROW_SHAPES + [(16384, 256, 256)]
+    probs, host = make(ctx, rng, shapes)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.cgemm_grouped(prec, probs)
+    prof = ctx.profile_read()
+    ctx.profile(False)
+    assert prof["cgemm_small"]["launches"] == 1
+    for (m, n, k, A, B, C), (a, b) in zip(probs, host):
+        got = ctx.download(C, np.complex64, (m, n))
+        single = ctx.alloc(m * n * 8)
+        ctx.cgemm(prec, m, n, k, A, B, single)
+        assert np.array_equal(ctx.download(single, np.complex64, (m, n)), got), ((m, n, k), prec)
+        single.free()
+        ref = a.astype(np.complex128) @ b.astype(np.complex128)
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= TOL[prec], (m, n, k)
+    free(probs)
