@@ -64,9 +64,19 @@ std::string strip_line(std::string line, bool join_paren) {
   return line;
 }
 
-std::vector<std::string> tokenize(const std::string& line, const std::string& expr = "[^\\s]+") {
+// Token patterns, compiled once: words of a line, and the parameters of
+// "gate(p1,p2)".
+const std::regex& word_pattern() {
+  static const std::regex word("[^\\s]+");
+  return word;
+}
+const std::regex& param_pattern() {
+  static const std::regex param("[^,]+");
+  return param;
+}
+
+std::vector<std::string> tokenize(const std::string& line, const std::regex& word = word_pattern()) {
   std::vector<std::string> tokens;
-  const std::regex word(expr);
   for (auto w = std::sregex_iterator(line.begin(), line.end(), word); w != std::sregex_iterator();
        ++w)
     tokens.push_back(w->str());
@@ -112,7 +122,7 @@ void QflexCircuit::load(std::istream&& istream) {
         if (auto beg = tokens[1].find_first_of('('); beg != std::string::npos) {
           gate.name = tokens[1].substr(0, beg);
           std::string params = tokens[1].substr(beg + 1, tokens[1].find_first_of(')') - beg - 1);
-          for (const auto& p : tokenize(params, "[^,]+")) {
+          for (const auto& p : tokenize(params, param_pattern())) {
             if (!is_number(p)) throw ERROR_MSG("Params must be valid numbers.");
             gate.params.push_back(std::stod(p));
           }
