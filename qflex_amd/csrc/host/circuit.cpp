@@ -1,12 +1,14 @@
-// Circuit parser; accepted format and error texts follow reference
-// src/circuit.cpp:30-206.
+// Circuit text parser.  The accepted text and the error messages are the
+// reference's contract (src/circuit.cpp:30-206, docs/input_formats.md); the
+// scanner below is a single pass over each line instead of a chain of
+// regular-expression rewrites.
 #include "circuit.h"
 
-#include <algorithm>
+#include <cerrno>
+#include <cstdlib>
 #include <fstream>
-#include <regex>
-#include <unordered_map>
-#include <unordered_set>
+#include <map>
+#include <set>
 
 #include "errors.h"
 
@@ -36,129 +38,177 @@ void QflexCircuit::load(std::istream& istream) { load(std::move(istream)); }
 
 namespace {
 
-bool is_number(const std::string& token) {
-  try {
-    std::stol(token);
-  } catch (...) {
-    return false;
+bool is_space(char c) {
+  return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
+}
+
+bool is_kept(char c) {
+  return is_space(c) || c == '(' || c == ')' || c == '_' || c == '.' || c == ',' || c == '-' ||
+         (c >= 'a' && c <= 'z') || (c >= 'A' && c <= 'Z') || (c >= '0' && c <= '9');
+}
+
+// Canonical form of one line of circuit or ordering text:
+//  * everything from '#' on is a comment;
+//  * characters outside [A-Za-z0-9_.,-()] and white space are dropped;
+//  * a run of two or more white-space characters becomes one space (a lone
+//    tab becomes a space), trailing white space goes;
+//  * with `glue_parens`, white space right before '(' goes;
+//  * white space whose next parenthesis is ')' (inside "( ... )") goes.
+// So "  2\tfsim( 0.25 , 0.1 )  0 1  # c" reads "2 fsim(0.25,0.1) 0 1" (the
+// leading space survives as one space; tokens ignore it).
+std::string canonical_line(const std::string& raw, bool glue_parens) {
+  std::string s;
+  s.reserve(raw.size());
+  for (char c : raw) {
+    if (c == '#') break;
+    if (is_kept(c)) s.push_back(c == '\t' ? ' ' : c);
   }
+  // collapse white-space runs
+  std::string t;
+  t.reserve(s.size());
+  for (std::size_t i = 0; i < s.size();) {
+    if (!is_space(s[i])) {
+      t.push_back(s[i++]);
+      continue;
+    }
+    std::size_t j = i;
+    while (j < s.size() && is_space(s[j])) ++j;
+    t.push_back(j - i >= 2 ? ' ' : s[i]);
+    i = j;
+  }
+  while (!t.empty() && is_space(t.back())) t.pop_back();
+  // drop white space before '(' and white space inside parentheses: scan
+  // from the right remembering the next parenthesis seen
+  std::string out(t.size(), '\0');
+  std::size_t w = t.size();
+  char next_paren = 0;
+  for (std::size_t i = t.size(); i-- > 0;) {
+    const char c = t[i];
+    // (white-space runs are single characters here)
+    if (is_space(c) && ((glue_parens && i + 1 < t.size() && t[i + 1] == '(') ||
+                        next_paren == ')'))
+      continue;
+    if (c == '(' || c == ')') next_paren = c;
+    out[--w] = c;
+  }
+  return out.substr(w);
+}
+
+std::vector<std::string> split_words(const std::string& s, char sep_if_any = 0) {
+  std::vector<std::string> words;
+  std::string cur;
+  for (char c : s) {
+    const bool sep = sep_if_any ? c == sep_if_any : is_space(c);
+    if (sep) {
+      if (!cur.empty()) words.push_back(cur);
+      cur.clear();
+    } else {
+      cur.push_back(c);
+    }
+  }
+  if (!cur.empty()) words.push_back(cur);
+  return words;
+}
+
+// A token "is a number" when it starts with a decimal integer (an optional
+// sign, then digits) that fits a long; "is an integer" when that integer
+// also equals the token read as a double ("3", "3.0", "3x"; not "3.5").
+bool leading_long(const std::string& tok, long* value) {
+  const char* s = tok.c_str();
+  char* end = nullptr;
+  errno = 0;
+  const long v = std::strtol(s, &end, 10);
+  if (end == s || errno == ERANGE) return false;
+  if (value) *value = v;
   return true;
 }
 
-bool is_integer(const std::string& token) {
-  return is_number(token) && double(std::stol(token)) == std::stod(token);
-}
-
-// Normalises a line to "cycle gate(p1,p2) q1 q2".
-std::string strip_line(std::string line, bool join_paren) {
-  static const std::regex comment("#.*"), special("[^)(\\s\\ta-zA-Z0-9_.,-]"), tab("[\\t]"),
-      spaces("[\\s]{2,}"), trailing("\\s+$"), before_paren("[\\s]+[(]"),
-      inside_paren("\\s+(?=[^()]*\\))");
-  line = std::regex_replace(line, comment, "");
-  line = std::regex_replace(line, special, "");
-  line = std::regex_replace(line, tab, " ");
-  line = std::regex_replace(line, spaces, " ");
-  line = std::regex_replace(line, trailing, "");
-  if (join_paren) line = std::regex_replace(line, before_paren, "(");
-  line = std::regex_replace(line, inside_paren, "");
-  return line;
-}
-
-// Token patterns, compiled once: words of a line, and the parameters of
-// "gate(p1,p2)".
-const std::regex& word_pattern() {
-  static const std::regex word("[^\\s]+");
-  return word;
-}
-const std::regex& param_pattern() {
-  static const std::regex param("[^,]+");
-  return param;
-}
-
-std::vector<std::string> tokenize(const std::string& line, const std::regex& word = word_pattern()) {
-  std::vector<std::string> tokens;
-  for (auto w = std::sregex_iterator(line.begin(), line.end(), word); w != std::sregex_iterator();
-       ++w)
-    tokens.push_back(w->str());
-  return tokens;
+bool integer_token(const std::string& tok, long* value) {
+  long v = 0;
+  if (!leading_long(tok, &v)) return false;
+  if (double(v) != std::strtod(tok.c_str(), nullptr)) return false;
+  if (value) *value = v;
+  return true;
 }
 
 }  // namespace
 
-std::string strip_ordering_line(const std::string& line) { return strip_line(line, false); }
+std::string strip_ordering_line(const std::string& line) { return canonical_line(line, false); }
+
+std::string canonical_circuit_line(const std::string& line) { return canonical_line(line, true); }
 
 void QflexCircuit::load(std::istream&& istream) {
   clear();
-  std::size_t line_counter = 0, last_cycle_number = 0;
-  std::unordered_set<std::size_t> used_qubits;
   std::string line;
-  auto error_msg = [&line, &line_counter](const std::string& msg) {
-    return ERROR_MSG("[", std::to_string(line_counter + 1), ": ", line, "] ", msg);
+  std::size_t line_no = 0;  // physical lines read so far, comments included
+  auto fail = [&](const std::string& msg) {
+    return ERROR_MSG("[", std::to_string(line_no + 1), ": ", line, "] ", msg);
   };
-  while (std::getline(istream, line)) {
-    if (!(line = strip_line(line, true)).empty()) {
-      const auto n_open = std::count(line.begin(), line.end(), '(');
-      const auto n_close = std::count(line.begin(), line.end(), ')');
-      if (n_open != n_close || n_open > 1) throw error_msg("Not-matching parentheses.");
-      auto tokens = tokenize(line);
-      if (line_counter == 0) {
-        if (tokens.size() != 1 || !is_number(tokens[0]) || std::stol(tokens[0]) <= 0)
-          throw error_msg("First line in circuit must be the number of active qubits.");
-        num_active_qubits = std::stol(tokens[0]);
-      } else {
-        if (tokens.size() < 3)
-          throw error_msg(
-              "Gate must be specified as: cycle gate_name[(p1[,p2,...])] q1 [q2, ...]");
-        gates.emplace_back();
-        auto& gate = gates.back();
-        gate.raw = line;
-        if (!is_integer(tokens[0])) throw error_msg("First token must be a valid cycle number.");
-        gate.cycle = std::stol(tokens[0]);
-        if (gate.cycle < last_cycle_number) throw error_msg("Cycle number can only increase.");
-        if (gate.cycle != last_cycle_number) {
-          used_qubits.clear();
-          last_cycle_number = gate.cycle;
-        }
-        if (auto beg = tokens[1].find_first_of('('); beg != std::string::npos) {
-          gate.name = tokens[1].substr(0, beg);
-          std::string params = tokens[1].substr(beg + 1, tokens[1].find_first_of(')') - beg - 1);
-          for (const auto& p : tokenize(params, param_pattern())) {
-            if (!is_number(p)) throw ERROR_MSG("Params must be valid numbers.");
-            gate.params.push_back(std::stod(p));
-          }
-        } else {
-          gate.name = tokens[1];
-        }
-        for (std::size_t i = 2; i < tokens.size(); ++i) {
-          if (!is_integer(tokens[i])) throw error_msg("Qubit must be a valid number.");
-          gate.qubits.push_back(std::stol(tokens[i]));
-        }
-        for (const auto& q : gate.qubits)
-          if (!used_qubits.insert(q).second)
-            throw error_msg("Qubits can only used once per cycle.");
+  std::size_t cycle_now = 0;
+  std::set<std::size_t> busy;  // qubits used in cycle_now
+  for (; std::getline(istream, line); ++line_no) {
+    line = canonical_line(line, true);
+    if (line.empty()) continue;
+    std::size_t opens = 0, closes = 0;
+    for (char c : line) opens += c == '(', closes += c == ')';
+    if (opens != closes || opens > 1) throw fail("Not-matching parentheses.");
+    const std::vector<std::string> words = split_words(line);
+    if (line_no == 0) {  // the first physical line holds the qubit count
+      long n = 0;
+      if (words.size() != 1 || !leading_long(words[0], &n) || n <= 0)
+        throw fail("First line in circuit must be the number of active qubits.");
+      num_active_qubits = std::size_t(n);
+      continue;
+    }
+    if (words.size() < 3)
+      throw fail("Gate must be specified as: cycle gate_name[(p1[,p2,...])] q1 [q2, ...]");
+    QflexGate gate;
+    gate.raw = line;
+    long cycle = 0;
+    if (!integer_token(words[0], &cycle)) throw fail("First token must be a valid cycle number.");
+    gate.cycle = std::size_t(cycle);
+    if (gate.cycle < cycle_now) throw fail("Cycle number can only increase.");
+    if (gate.cycle != cycle_now) {
+      busy.clear();
+      cycle_now = gate.cycle;
+    }
+    const std::string& head = words[1];
+    const std::size_t open = head.find('(');
+    gate.name = head.substr(0, open);
+    if (open != std::string::npos) {
+      const std::size_t close = head.find(')');
+      for (const auto& p : split_words(head.substr(open + 1, close - open - 1), ',')) {
+        if (!leading_long(p, nullptr)) throw ERROR_MSG("Params must be valid numbers.");
+        gate.params.push_back(std::strtod(p.c_str(), nullptr));
       }
     }
-    ++line_counter;
+    for (std::size_t w = 2; w < words.size(); ++w) {
+      long q = 0;
+      if (!integer_token(words[w], &q)) throw fail("Qubit must be a valid number.");
+      gate.qubits.push_back(std::size_t(q));
+    }
+    gates.push_back(gate);
+    for (auto q : gates.back().qubits)
+      if (!busy.insert(q).second) throw fail("Qubits can only used once per cycle.");
   }
-  // Depth: largest number of two-qubit gates on one pair (fsim counts 2).
-  std::unordered_map<std::size_t, std::unordered_map<std::size_t, std::size_t>> layers;
+  // Depth = the largest number of two-qubit gates acting on one qubit pair
+  // (an fsim counts twice: its virtual bond has dimension 4).
+  std::map<std::pair<std::size_t, std::size_t>, std::size_t> per_pair;
   for (const auto& gate : gates) {
     if (gate.qubits.size() > 2)
-      throw error_msg("Depth calculation does not handle k-qubit gates with k > 2.");
-    if (gate.qubits.size() == 2) {
-      auto q1 = gate.qubits[0], q2 = gate.qubits[1];
-      if (q2 > q1) std::swap(q1, q2);
-      layers[q1][q2] += (gate.name == "fsim") ? 2 : 1;
-      depth = std::max(depth, layers[q1][q2]);
-    }
+      throw fail("Depth calculation does not handle k-qubit gates with k > 2.");
+    if (gate.qubits.size() != 2) continue;
+    const std::size_t a = gate.qubits[0], b = gate.qubits[1];
+    std::size_t& d = per_pair[a < b ? std::make_pair(a, b) : std::make_pair(b, a)];
+    d += gate.name == "fsim" ? 2 : 1;
+    if (d > depth) depth = d;
   }
 }
 
 void QflexCircuit::load(const std::string& filename) {
-  if (auto in = std::ifstream(filename); in.good())
-    load(in);
-  else
-    throw ERROR_MSG("Cannot open circuit file ", filename, ".");
+  std::ifstream in(filename);
+  if (!in.good()) throw ERROR_MSG("Cannot open circuit file ", filename, ".");
+  load(in);
 }
 
 }  // namespace qflex

@@ -34,6 +34,12 @@ struct QflexCircuit {
   void load(const std::string& filename);
 };
 
+// Canonical form of one line of circuit text: what the parser splits into
+// words and what its error messages quote.
+std::string canonical_circuit_line(const std::string& line);
+// The same for ordering text (no gluing of "name (" into "name(").
+std::string strip_ordering_line(const std::string& line);
+
 }  // namespace qflex
 
 #endif

@@ -7,7 +7,10 @@
 #include <algorithm>
 #include <chrono>
 #include <iostream>
-#include <regex>
+#include <cctype>
+#include <cstdlib>
+#include <map>
+#include <set>
 #include <sstream>
 #include <unordered_set>
 
@@ -83,69 +86,96 @@ bool find_grid_coord_in_list(const std::optional<std::vector<std::vector<std::si
 
 // ------------------------------------------------------------ parsing
 
+namespace {
+
+// The words of an ordering line: the operation, then its arguments.
+std::vector<std::string> words_of(const std::string& line) {
+  std::vector<std::string> w;
+  std::size_t i = 0;
+  while (i < line.size()) {
+    while (i < line.size() && std::isspace(static_cast<unsigned char>(line[i]))) ++i;
+    const std::size_t b = i;
+    while (i < line.size() && !std::isspace(static_cast<unsigned char>(line[i]))) ++i;
+    if (i > b) w.push_back(line.substr(b, i - b));
+  }
+  return w;
+}
+
+// A qubit number as stream extraction reads it: a missing word gives
+// `missing`, a word without a leading integer gives 0, otherwise its leading
+// integer ("7", "7x" -> 7).
+long qubit_word(const std::vector<std::string>& w, std::size_t at, long missing) {
+  if (at >= w.size()) return missing;
+  const char* s = w[at].c_str();
+  char* end = nullptr;
+  const long v = std::strtol(s, &end, 10);
+  return end == s ? 0 : v;
+}
+
+// "(v1,v2,...)" with digits and commas only, e.g. "()" or "(0,1,3)"; empty
+// pieces read as 0.  False when the word has another form.
+bool cut_values(const std::string& word, std::vector<std::size_t>* values) {
+  if (word.size() < 2 || word.front() != '(' || word.back() != ')') return false;
+  for (std::size_t i = 1; i + 1 < word.size(); ++i)
+    if (!std::isdigit(static_cast<unsigned char>(word[i])) && word[i] != ',') return false;
+  if (word.size() == 2) return true;
+  std::size_t v = 0;
+  for (std::size_t i = 1; i < word.size(); ++i) {
+    if (word[i] == ',' || word[i] == ')') {
+      values->push_back(v);
+      v = 0;
+    } else {
+      v = 10 * v + std::size_t(word[i] - '0');
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
 void ordering_data_to_contraction_ordering(const QflexInput& input,
                                            std::list<ContractionOperation>* ordering) {
   if (ordering == nullptr) throw ERROR_MSG("Ordering must be non-null.");
-  static const std::regex cut_value_regex("\\([0-9,]*\\)");
-  std::string operation;
+  const std::size_t sites = input.grid.I * input.grid.J;
   for (const auto& line : input.ordering.instructions) {
-    auto error_msg = [&line](const auto& msg) {
+    if (line.empty() || line[0] == '#') continue;
+    auto fail = [&line](const std::string& msg) {
       return ERROR_MSG("Parsing failed on line: '", line, "' with error: ", msg);
     };
-    auto check_index = [&input, &error_msg](long int index) {
-      const std::size_t grid_size = input.grid.I * input.grid.J;
-      if (index < 0 || std::size_t(index) >= grid_size)
-        throw error_msg(utils::concat("Index ", index, " must be within grid boundaries."));
+    // Grid coordinates of qubit q, which must be on the grid and active;
+    // `which` names the qubit in the error text ("", "1 ", "2 ").
+    auto site = [&](long q, const char* which) {
+      if (q < 0 || std::size_t(q) >= sites)
+        throw fail(utils::concat("Index ", q, " must be within grid boundaries."));
+      const std::vector<std::size_t> at = get_qubit_coords(std::size_t(q), input.grid.J);
+      if (find_grid_coord_in_list(input.grid.qubits_off, at[0], at[1]))
+        throw fail(utils::concat("Index ", which, "must specify an active qubit."));
+      return at;
     };
-    if (line.empty() || line[0] == '#') continue;
-    std::stringstream ss(line);
-    ss >> operation;
-    if (operation == "expand") {
-      std::string patch;
-      long int index = -1;
-      ss >> patch >> index;
-      check_index(index);
-      auto position = get_qubit_coords(index, input.grid.J);
-      if (find_grid_coord_in_list(input.grid.qubits_off, position[0], position[1]))
-        throw error_msg("Index must specify an active qubit.");
-      ordering->emplace_back(ExpandPatch(patch, position));
-    } else if (operation == "cut") {
+    const std::vector<std::string> w = words_of(line);
+    const std::string op = w.empty() ? std::string() : w[0];
+    const std::string arg1 = w.size() > 1 ? w[1] : std::string();
+    if (op == "expand") {
+      ordering->emplace_back(ExpandPatch(arg1, site(qubit_word(w, 2, -1), "")));
+    } else if (op == "merge") {
+      ordering->emplace_back(MergePatches(arg1, w.size() > 2 ? w[2] : std::string()));
+    } else if (op == "cut") {
       std::vector<std::size_t> values;
-      std::string values_str;
-      ss >> values_str;
-      if (!std::regex_match(values_str, cut_value_regex))
-        throw error_msg("Cut values must be comma-separated ints, e.g. (0,1,3).");
-      values_str = values_str.substr(1, values_str.size() - 2);
-      if (!values_str.empty()) {
-        std::stringstream vs(values_str);
-        std::string v;
-        while (std::getline(vs, v, ',')) values.push_back(v.empty() ? 0 : std::stoul(v));
+      if (!cut_values(arg1, &values))
+        throw fail("Cut values must be comma-separated ints, e.g. (0,1,3).");
+      const long q1 = qubit_word(w, 2, -1);
+      auto p1 = site(q1, "1 ");
+      if (w.size() <= 3) {  // a terminal cut: the output index of one qubit
+        ordering->emplace_back(CutIndex({p1}, values));
+        continue;
       }
-      long int index_1 = -1, index_2 = -1;
-      ss >> index_1;
-      check_index(index_1);
-      auto position_1 = get_qubit_coords(index_1, input.grid.J);
-      if (find_grid_coord_in_list(input.grid.qubits_off, position_1[0], position_1[1]))
-        throw error_msg("Index 1 must specify an active qubit.");
-      if (ss.eof()) {
-        ordering->emplace_back(CutIndex({position_1}, values));
-      } else {
-        ss >> index_2;
-        check_index(index_2);
-        auto position_2 = get_qubit_coords(index_2, input.grid.J);
-        if (find_grid_coord_in_list(input.grid.qubits_off, position_2[0], position_2[1]))
-          throw error_msg("Index 2 must specify an active qubit.");
-        if (index_1 < index_2)
-          ordering->emplace_back(CutIndex({position_1, position_2}, values));
-        else
-          ordering->emplace_back(CutIndex({position_2, position_1}, values));
-      }
-    } else if (operation == "merge") {
-      std::string patch_1, patch_2;
-      ss >> patch_1 >> patch_2;
-      ordering->emplace_back(MergePatches(patch_1, patch_2));
+      const long q2 = qubit_word(w, 3, -1);
+      auto p2 = site(q2, "2 ");
+      // the bond is named from its lower-numbered end
+      if (q2 < q1) std::swap(p1, p2);
+      ordering->emplace_back(CutIndex({p1, p2}, values));
     } else {
-      throw error_msg("Received an invalid operation in config.");
+      throw fail("Received an invalid operation in config.");
     }
   }
   ValidateOrdering(*ordering);
@@ -153,57 +183,54 @@ void ordering_data_to_contraction_ordering(const QflexInput& input,
 
 // ------------------------------------------------------------ validation
 
+// Replays the ordering's patch bookkeeping and collects every violation in
+// one message (thrown as a plain string, like the reference's
+// src/contraction_utils.cpp:534-610, whose message texts are kept):
+//  * a grid tensor is contracted at most once and a bond is cut at most once;
+//  * a patch grown before a cut is frozen: after the cut nothing may be added
+//    to it and nothing merged into it (it is reused across the cut values);
+//  * a patch merged into another one is gone: it can be neither merged again
+//    nor grown.
 void ValidateOrdering(const std::list<ContractionOperation>& ordering) {
-  struct PatchState {
-    bool is_active = false;  // expanded, no cut since
-    bool is_used = false;    // expanded, and a cut happened since
-    bool is_merged = false;  // merged into another patch
-  };
-  std::unordered_map<std::string, PatchState> patches;
-  std::unordered_set<std::string> cut_indices, used_tensors;
-  std::string error_msg;
+  enum : unsigned { kGrown = 1, kFrozen = 2, kGone = 4 };
+  std::map<std::string, unsigned> patch;  // patch id -> state bits
+  std::set<std::pair<std::size_t, std::size_t>> contracted;
+  std::set<std::string> cut_bonds;
+  std::ostringstream problems;
   for (const auto& op : ordering) {
-    switch (op.op_type) {
-      case ContractionOperation::EXPAND: {
-        auto& st = patches[op.expand.id];
-        if (st.is_used || st.is_merged)
-          error_msg = utils::concat(error_msg, "\nTensor at (", op.expand.tensor[0], ",",
-                                    op.expand.tensor[1], ") is added to non-empty patch ",
-                                    op.expand.id, " after a cut.");
-        std::string name = utils::concat("(", op.expand.tensor[0], ",", op.expand.tensor[1], ")");
-        if (!used_tensors.insert(name).second)
-          error_msg = utils::concat(error_msg, "\nTensor ", name, " is contracted multiple times.");
-        st.is_active = true;
-        break;
+    if (op.op_type == ContractionOperation::CUT) {
+      for (auto& p : patch)
+        if (p.second & kGrown) p.second |= kFrozen;
+      std::string bond;
+      try {
+        bond = index_name(op.cut.tensors);
+      } catch (const std::string& err_msg) {
+        throw ERROR_MSG("Failed to during CUT. Error:\n\t[", err_msg, "]");
       }
-      case ContractionOperation::CUT: {
-        for (auto& p : patches)
-          if (p.second.is_active) p.second.is_used = true;
-        std::string index;
-        try {
-          index = index_name(op.cut.tensors);
-        } catch (const std::string& err_msg) {
-          throw ERROR_MSG("Failed to during CUT. Error:\n\t[", err_msg, "]");
-        }
-        if (!cut_indices.insert(index).second)
-          error_msg = utils::concat(error_msg, "\nIndex ", index, " is cut multiple times.");
-        break;
-      }
-      case ContractionOperation::MERGE: {
-        if (patches[op.merge.source_id].is_merged)
-          error_msg = utils::concat(error_msg, "\nPatch ", op.merge.source_id,
-                                    " is merged multiple times.");
-        if (patches[op.merge.target_id].is_used)
-          error_msg = utils::concat(error_msg, "\nPatch ", op.merge.source_id,
-                                    " is merged into non-empty patch ", op.merge.target_id,
-                                    " after a cut.");
-        patches[op.merge.source_id].is_merged = true;
-        patches[op.merge.target_id].is_active = true;
-        break;
-      }
+      if (!cut_bonds.insert(bond).second)
+        problems << "\nIndex " << bond << " is cut multiple times.";
+    } else if (op.op_type == ContractionOperation::EXPAND) {
+      const std::size_t i = op.expand.tensor[0], j = op.expand.tensor[1];
+      unsigned& st = patch[op.expand.id];
+      for (unsigned bit : {unsigned(kFrozen), unsigned(kGone)})
+        if (st & bit)
+          problems << "\nTensor at (" << i << "," << j << ") is added to non-empty patch "
+                   << op.expand.id << " after a cut.";
+      if (!contracted.insert({i, j}).second)
+        problems << "\nTensor (" << i << "," << j << ") is contracted multiple times.";
+      st |= kGrown;
+    } else {
+      const std::string& from = op.merge.source_id;
+      const std::string& into = op.merge.target_id;
+      if (patch[from] & kGone) problems << "\nPatch " << from << " is merged multiple times.";
+      if (patch[into] & kFrozen)
+        problems << "\nPatch " << from << " is merged into non-empty patch " << into
+                 << " after a cut.";
+      patch[from] |= kGone;
+      patch[into] |= kGrown;
     }
   }
-  if (!error_msg.empty()) throw error_msg;
+  if (!problems.str().empty()) throw problems.str();
 }
 
 // ------------------------------------------------------------ planning

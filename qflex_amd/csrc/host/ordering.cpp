@@ -3,9 +3,9 @@
 
 #include <fstream>
 
-namespace qflex {
+#include "circuit.h"
 
-std::string strip_ordering_line(const std::string& line);  // circuit.cpp
+namespace qflex {
 
 void QflexOrdering::clear() { instructions.clear(); }
 

@@ -28,58 +28,56 @@ namespace py = pybind11;
 
 namespace {
 
-std::stringstream GetStream(const py::iterable& data, const std::string& caller = "") {
-  std::stringstream out;
-  for (const auto& line : data)
-    if (py::isinstance<py::str>(line))
-      out << line.cast<std::string>() << std::endl;
-    else
-      throw ERROR_MSG(caller, " must be a list of strings.");
-  return out;
+// The lines of a Python list of strings, newline-terminated, as one stream.
+std::stringstream lines_of(const py::handle& seq, const std::string& key) {
+  std::stringstream text;
+  for (const auto& item : seq.cast<py::iterable>()) {
+    if (!py::isinstance<py::str>(item)) throw ERROR_MSG(key, " must be a list of strings.");
+    text << item.cast<std::string>() << '\n';
+  }
+  return text;
 }
 
-template <typename data_type>
-void LoadData(const py::dict& options, const std::string& argument, data_type& data) {
-  if (options.contains(argument.c_str())) {
-    const auto& it = options[argument.c_str()];
-    if (py::isinstance<py::iterable>(it) && !py::isinstance<py::str>(it))
-      data.load(GetStream(it.cast<py::iterable>(), argument));
-    else
-      throw ERROR_MSG("'", argument, "' must be a list of strings.");
-  } else if (options.contains((argument + "_filename").c_str())) {
-    const auto& filename = options[(argument + "_filename").c_str()];
-    if (py::isinstance<py::str>(filename))
-      data.load(filename.cast<std::string>());
-    else
-      throw ERROR_MSG("'", argument, "_filename' must be a string.");
+// Input `key` ("grid", "circuit" or "ordering") of a simulate() dict: the
+// text itself as a list of lines under `key`, else a file path under
+// "<key>_filename"; with neither, `target` stays empty.  A bare string under
+// `key` is refused (it would otherwise be read one character per line).
+template <typename Target>
+void read_input(const py::dict& options, const std::string& key, Target& target) {
+  if (options.contains(key)) {
+    const py::handle value = options[key.c_str()];
+    if (py::isinstance<py::str>(value) || !py::isinstance<py::iterable>(value))
+      throw ERROR_MSG("'", key, "' must be a list of strings.");
+    target.load(lines_of(value, key));
+    return;
   }
+  const std::string file_key = key + "_filename";
+  if (!options.contains(file_key)) return;
+  const py::handle path = options[file_key.c_str()];
+  if (!py::isinstance<py::str>(path)) throw ERROR_MSG("'", file_key, "' must be a string.");
+  target.load(path.cast<std::string>());
 }
 
-std::vector<std::string> LoadStates(const py::dict& options, const std::string& argument) {
-  std::vector<std::string> states;
-  if (options.contains((argument + "_states").c_str())) {
-    const auto& in = options[(argument + "_states").c_str()];
-    if (py::isinstance<py::str>(in)) {
-      states.push_back(in.cast<std::string>());
-    } else if (py::isinstance<py::iterable>(in)) {
-      for (const auto& x : in.cast<py::iterable>())
-        if (py::isinstance<py::str>(x))
-          states.push_back(x.cast<std::string>());
-        else
-          throw ERROR_MSG("states must be strings.");
-    } else {
-      throw ERROR_MSG(argument, "_states must be a list of strings.");
-    }
-  } else if (options.contains((argument + "_state").c_str())) {
-    const auto& in = options[(argument + "_state").c_str()];
-    if (py::isinstance<py::str>(in))
-      states.push_back(in.cast<std::string>());
-    else
-      throw ERROR_MSG("states must be strings.");
-  } else {
-    states.push_back("");
+// Bitstrings of one side ("initial" or "final"): "<side>_states" holds one
+// string or a list of them, "<side>_state" one string; neither gives the
+// empty string (no fixed bits).
+std::vector<std::string> read_states(const py::dict& options, const std::string& side) {
+  auto one_string = [](const py::handle& h) {
+    if (!py::isinstance<py::str>(h)) throw ERROR_MSG("states must be strings.");
+    return h.cast<std::string>();
+  };
+  const std::string plural = side + "_states", singular = side + "_state";
+  if (options.contains(plural)) {
+    const py::handle value = options[plural.c_str()];
+    if (py::isinstance<py::str>(value)) return {value.cast<std::string>()};
+    if (!py::isinstance<py::iterable>(value))
+      throw ERROR_MSG(plural, " must be a list of strings.");
+    std::vector<std::string> states;
+    for (const auto& item : value.cast<py::iterable>()) states.push_back(one_string(item));
+    return states;
   }
-  return states;
+  if (options.contains(singular)) return {one_string(options[singular.c_str()])};
+  return {""};
 }
 
 int parse_precision(const std::string& p) {
@@ -123,11 +121,11 @@ void load_input(const py::dict& options, qflex::QflexInput& input) {
     qflex::qfx_check(qfx_autotune_enable(qflex::device(), options["autotune"].cast<bool>() ? 1 : 0),
                      "qfx_autotune_enable");
 
-  LoadData(options, "grid", input.grid);
-  LoadData(options, "circuit", input.circuit);
-  LoadData(options, "ordering", input.ordering);
-  const auto initial_states = LoadStates(options, "initial");
-  const auto final_states = LoadStates(options, "final");
+  read_input(options, "grid", input.grid);
+  read_input(options, "circuit", input.circuit);
+  read_input(options, "ordering", input.ordering);
+  const auto initial_states = read_states(options, "initial");
+  const auto final_states = read_states(options, "final");
   if (initial_states.size() != 1 || final_states.size() != 1)
     throw ERROR_MSG("Not yet supported");
   input.initial_state = initial_states[0];
@@ -408,6 +406,12 @@ PYBIND11_MODULE(qflex, m) {
       for (const auto& g : c.gates) gates.append(py::make_tuple(g.cycle, g.name, g.qubits, g.params));
       return py::make_tuple(c.num_active_qubits, c.depth, gates);
     });
+  });
+  m.def("canonical_lines", [](std::vector<std::string> lines, bool circuit) {
+    std::vector<std::string> out;
+    for (const auto& l : lines)
+      out.push_back(circuit ? qflex::canonical_circuit_line(l) : qflex::strip_ordering_line(l));
+    return out;
   });
   m.def("load_grid", [](std::vector<std::string> lines) {
     return wrap([&] {

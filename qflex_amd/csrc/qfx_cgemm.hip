@@ -1745,6 +1745,9 @@ extern "C" int qfx_cgemv_c64(qfx_context* ctx, int trans, uint64_t m, uint64_t k
   if (ny == 0) return QFX_OK;
   if (!A || !x || !y) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemv_c64: null pointer");
   if ((trans ? m : k) == 0) return qfx_zero(ctx, y, ny * sizeof(qfx_c64));
+  QFX_EXTENT(ctx, A, 8.0 * double(m) * double(k), "qfx_cgemv_c64: A");
+  QFX_EXTENT(ctx, x, 8.0 * double(trans ? m : k), "qfx_cgemv_c64: x");
+  QFX_EXTENT(ctx, y, 8.0 * double(ny), "qfx_cgemv_c64: y");
   QfxLaunchScope scope(ctx, QFX_K_GEMV, 8.0 * (m * k + m + k), 8.0 * m * k);
   const float2* A2 = reinterpret_cast<const float2*>(A);
   const float2* x2 = reinterpret_cast<const float2*>(x);
@@ -1763,6 +1766,9 @@ extern "C" int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, con
   if (!out) return qfx_set_error(QFX_ERR_INVALID, "qfx_cdotu_c64: null output");
   if (n == 0) return qfx_zero(ctx, out, sizeof(qfx_c64));
   if (!x || !y) return qfx_set_error(QFX_ERR_INVALID, "qfx_cdotu_c64: null pointer");
+  QFX_EXTENT(ctx, x, 8.0 * double(n), "qfx_cdotu_c64: x");
+  QFX_EXTENT(ctx, y, 8.0 * double(n), "qfx_cdotu_c64: y");
+  QFX_EXTENT(ctx, out, 8.0, "qfx_cdotu_c64: out");
   QfxLaunchScope scope(ctx, QFX_K_DOT, 16.0 * n, 8.0 * n);
   uint64_t blocks = std::min<uint64_t>((n + 255) / 256, 1024);
   void* ws = nullptr;
@@ -1827,6 +1833,9 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   if (!C) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: null C");
   if (k == 0) return qfx_zero(ctx, C, m * n * sizeof(qfx_c64));
   if (!A || !B) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: null A or B");
+  QFX_EXTENT(ctx, A, 8.0 * double(m) * double(k), "qfx_cgemm_c64: A");
+  QFX_EXTENT(ctx, B, 8.0 * double(k) * double(n), "qfx_cgemm_c64: B");
+  QFX_EXTENT(ctx, C, 8.0 * double(m) * double(n), "qfx_cgemm_c64: C");
   const float2* A2 = reinterpret_cast<const float2*>(A);
   const float2* B2 = reinterpret_cast<const float2*>(B);
   float2* C2 = reinterpret_cast<float2*>(C);
@@ -1979,6 +1988,13 @@ static bool map_size(const qfx_index_map* d, uint64_t* size) {
   return true;
 }
 
+// Offset of the map's last index (every digit at its largest value).
+static uint64_t map_last(const qfx_index_map& d) {
+  uint64_t off = 0;
+  for (int j = 0; j < d.ndigits; ++j) off += ((uint64_t(1) << d.bits[j]) - 1) << d.shift[j];
+  return off;
+}
+
 // The map is the identity: its digits in order from stride 1, index i at
 // offset i.
 static bool identity_map(const qfx_index_map& d) {
@@ -2028,6 +2044,11 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
   if (k != k2)
     return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: a_cols and b_rows differ in size");
   if (!A || !B || !C) return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_mapped_c64: null pointer");
+  QFX_EXTENT(ctx, A, 8.0 * double(map_last(*a_rows) + map_last(*a_cols) + 1),
+             "qfx_cgemm_mapped_c64: A");
+  QFX_EXTENT(ctx, B, 8.0 * double(map_last(*b_rows) + map_last(*b_cols) + 1),
+             "qfx_cgemm_mapped_c64: B");
+  QFX_EXTENT(ctx, C, 8.0 * double(m) * double(n), "qfx_cgemm_mapped_c64: C");
   const float2* A2 = reinterpret_cast<const float2*>(A);
   const float2* B2 = reinterpret_cast<const float2*>(B);
   float2* C2 = reinterpret_cast<float2*>(C);

@@ -278,7 +278,8 @@ uint64_t round_up(uint64_t x, uint64_t r) { return (x + r - 1) / r * r; }
 }  // namespace
 
 bool qfx_fp8_applies(uint64_t m, uint64_t n, uint64_t k) {
-  return m >= 128 && n >= 64 && k >= 64 && m <= (uint64_t(1) << 40) &&
+  // the quantize grid has Mp / 128 blocks along x (below 2^31)
+  return m >= 128 && n >= 64 && k >= 64 && m <= (uint64_t(1) << 36) &&
          round_up(2 * n, kF8BN) / kF8BN <= 65535 && round_up(2 * k, kQT) / kQT <= 65535;
 }
 

@@ -124,6 +124,17 @@ struct qfx_graph {
 // Device workspace of at least `bytes` (reused across calls on the stream).
 int qfx_workspace(qfx_context* ctx, size_t bytes, void** out);
 
+// Host-side guard before a launch: when p lies in one of the context's arena
+// chunks, [p, p + bytes) must lie inside one live (allocated, not freed)
+// block; otherwise QFX_ERR_INVALID with a message naming `what`.  Memory
+// outside the arena (allocated by the caller) is not checked.
+int qfx_check_extent(qfx_context* ctx, const void* p, double bytes, const char* what);
+#define QFX_EXTENT(ctx, p, bytes, what)                        \
+  do {                                                         \
+    const int extent_status_ = qfx_check_extent(ctx, p, bytes, what); \
+    if (extent_status_ != QFX_OK) return extent_status_;       \
+  } while (0)
+
 // Offset of matrix index idx under a digit map (see qfx_index_map).  The
 // map is a bit permutation, so for i0 a multiple of 2^b and d < 2^b,
 // offset(i0 + d) = offset(i0) + offset(d).

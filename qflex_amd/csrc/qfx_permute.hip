@@ -219,6 +219,8 @@ extern "C" int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* ds
   }
   if (!src || !dst) return qfx_set_error(QFX_ERR_INVALID, "qfx_reorder_c64: null data pointer");
   if (src == dst) return qfx_set_error(QFX_ERR_INVALID, "qfx_reorder_c64: src and dst alias");
+  QFX_EXTENT(ctx, src, 8.0 * double(total), "qfx_reorder_c64: src");
+  QFX_EXTENT(ctx, dst, 8.0 * double(total), "qfx_reorder_c64: dst");
 
   // Drop unit axes, then fuse axes that remain adjacent in the output.
   std::vector<int> keep_map(rank, -1);

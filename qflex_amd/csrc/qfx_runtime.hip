@@ -147,6 +147,25 @@ extern "C" int qfx_arena_stats(qfx_context* ctx, size_t* reserved, size_t* in_us
   return QFX_OK;
 }
 
+int qfx_check_extent(qfx_context* ctx, const void* p, double bytes, const char* what) {
+  if (bytes <= 0) return QFX_OK;
+  const char* a = static_cast<const char*>(p);
+  bool in_arena = false;
+  for (const auto& c : ctx->chunks)
+    if (a >= c.base && a < c.base + c.size) in_arena = true;
+  if (!in_arena) return QFX_OK;  // memory the caller allocated elsewhere
+  auto it = ctx->live.upper_bound(const_cast<void*>(p));
+  if (it != ctx->live.begin()) {
+    --it;
+    const char* b0 = static_cast<const char*>(it->first);
+    if (a >= b0 && double(a - b0) + bytes <= double(it->second)) return QFX_OK;
+  }
+  return qfx_set_error(QFX_ERR_INVALID,
+                       std::string(what) + ": " + std::to_string(uint64_t(bytes)) +
+                           " bytes from the given address are not inside one live block of "
+                           "the context's device arena");
+}
+
 int qfx_workspace(qfx_context* ctx, size_t bytes, void** out) {
   if (ctx->work_bytes < bytes && ctx->capturing)
     return qfx_set_error(QFX_ERR_INVALID,
@@ -160,6 +179,7 @@ int qfx_workspace(qfx_context* ctx, size_t bytes, void** out) {
       QFX_HIP(hipStreamSynchronize(ctx->stream));
       qfx_free(ctx, ctx->work);
       ctx->work = nullptr;
+      ctx->work_bytes = 0;  // a failed allocation below leaves no workspace
     }
     size_t sz = std::max<size_t>(bytes, 1 << 20);
     int st = qfx_alloc(ctx, sz, &ctx->work);
@@ -409,6 +429,8 @@ extern "C" int qfx_copy_d2d(qfx_context* ctx, void* dst, const void* src, size_t
   QFX_CHECK_CTX(ctx);
   if (!bytes || dst == src) return QFX_OK;
   if (!dst || !src) return qfx_set_error(QFX_ERR_INVALID, "qfx_copy_d2d: null pointer");
+  QFX_EXTENT(ctx, src, double(bytes), "qfx_copy_d2d: src");
+  QFX_EXTENT(ctx, dst, double(bytes), "qfx_copy_d2d: dst");
   QfxLaunchScope scope(ctx, QFX_K_COPY, 2.0 * bytes, 0);
   if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src) | bytes) % 16 == 0) {
     uint64_t n16 = bytes / 16;
@@ -457,6 +479,7 @@ extern "C" int qfx_zero(qfx_context* ctx, void* dst, size_t bytes) {
   QFX_CHECK_CTX(ctx);
   if (!bytes) return QFX_OK;
   if (!dst) return qfx_set_error(QFX_ERR_INVALID, "qfx_zero: null pointer");
+  QFX_EXTENT(ctx, dst, double(bytes), "qfx_zero: dst");
   QfxLaunchScope scope(ctx, QFX_K_COPY, double(bytes), 0);
   if ((reinterpret_cast<uintptr_t>(dst) | bytes) % 16 == 0)
     qfx_zero16_kernel<<<grid_for(bytes / 16, 256), 256, 0, ctx->stream>>>(
@@ -495,6 +518,8 @@ extern "C" int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_
   QFX_CHECK_CTX(ctx);
   if (!n) return QFX_OK;
   if (!src || !amps) return qfx_set_error(QFX_ERR_INVALID, "qfx_accumulate_c128: null pointer");
+  QFX_EXTENT(ctx, src, 8.0 * double(n), "qfx_accumulate_c128: src");
+  QFX_EXTENT(ctx, amps + offset, 16.0 * double(n), "qfx_accumulate_c128: amps");
   QfxLaunchScope scope(ctx, QFX_K_ACCUMULATE, 8.0 * n + 32.0 * n, 2.0 * n);
   unsigned g = unsigned((n + 255) / 256);
   qfx_accumulate_kernel<<<g, 256, 0, ctx->stream>>>(reinterpret_cast<const float2*>(src), n,
@@ -575,6 +600,8 @@ extern "C" int qfx_project_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* ds
       return qfx_set_error(QFX_ERR_INVALID, "qfx_project_c64: index value " +
                                                 std::to_string(picks[j]) + " outside [0, " +
                                                 std::to_string(dim) + ")");
+  QFX_EXTENT(ctx, src, 8.0 * double(pre) * double(dim) * double(post), "qfx_project_c64: src");
+  QFX_EXTENT(ctx, dst, 8.0 * double(pre) * double(npicks) * double(post), "qfx_project_c64: dst");
   QfxLaunchScope scope(ctx, QFX_K_COPY, 16.0 * pre * npicks * post, 0);
   for (uint64_t j0 = 0; j0 < npicks; j0 += 64) {
     const uint32_t npk = uint32_t(std::min<uint64_t>(64, npicks - j0));

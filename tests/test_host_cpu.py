@@ -146,6 +146,152 @@ def test_simulate_rejects_bad_memory_limit(capfd):
     assert "'memory_limit' must be positive." in capfd.readouterr().err
 
 
+# ---------------------------------------------------------------- text formats in depth
+
+def reference_line_rules(line, glue_parens):
+    """The line normalisation that src/circuit.cpp:47-73 (circuit text, with
+    glue_parens) and src/ordering.cpp:12-38 (ordering text) state as a
+    sequence of regular-expression rewrites, applied here with Python's re
+    in ASCII mode (the character classes of std::regex's ECMAScript
+    grammar).  The C++ parser implements the same contract with its own
+    scanner; this is the independent statement it is checked against."""
+    import re
+    A = re.ASCII
+    line = re.sub(r"#.*", "", line, flags=A)
+    line = re.sub(r"[^)(\s\ta-zA-Z0-9_.,-]", "", line, flags=A)
+    line = re.sub(r"[\t]", " ", line, flags=A)
+    line = re.sub(r"[\s]{2,}", " ", line, flags=A)
+    line = re.sub(r"\s+$", "", line, flags=A)
+    if glue_parens:
+        line = re.sub(r"[\s]+[(]", "(", line, flags=A)
+    line = re.sub(r"\s+(?=[^()]*\))", "", line, flags=A)
+    return line
+
+
+@pytest.mark.parametrize("glue", [True, False])
+def test_canonical_lines_match_the_reference_rewrite_rules(glue):
+    rng = np.random.default_rng(7 if glue else 8)
+    alphabet = list(" \t\r()#,.-_aZx09$%;") + ["  ", "\t\t", " (", ") ", "fsim", "cz"]
+    lines = ["  2\tfsim( 0.25 , 0.1 )  0 1  # c", "0 h 0", "\t\t", "cut   (  1  , 2   )  2  3",
+             "a (b (c) d) e", "x ) y ( z", " (", "(  )", "1 rz (0.5) 3\r"]
+    for _ in range(3000):
+        n = int(rng.integers(0, 14))
+        lines.append("".join(alphabet[int(i)] for i in rng.integers(0, len(alphabet), n)))
+    got = qflex.canonical_lines(lines, glue)
+    want = [reference_line_rules(l, glue) for l in lines]
+    bad = [(l, g, w) for l, g, w in zip(lines, got, want) if g != w]
+    assert not bad, bad[:5]
+
+
+MORE_BAD_CIRCUITS = [
+    ("2\n1.5 h 0\n", "[2: 1.5 h 0] First token must be a valid cycle number."),
+    ("2\n1 h 0.5\n", "[2: 1 h 0.5] Qubit must be a valid number."),
+    ("2\n1 rz(.5) 0\n", "Params must be valid numbers."),
+    ("3\n1 ccz 0 1 2\n", "Depth calculation does not handle k-qubit gates with k > 2."),
+    ("2\n1 h((0) 0\n", "[2: 1 h((0) 0] Not-matching parentheses."),
+    ("2\n1 rz(0.1)(0.2) 0\n", "Not-matching parentheses."),
+    ("x\n", "[1: x] First line in circuit must be the number of active qubits."),
+    ("0\n", "[1: 0] First line in circuit must be the number of active qubits."),
+    ("2 3\n", "[1: 2 3] First line in circuit must be the number of active qubits."),
+]
+
+
+@pytest.mark.parametrize("text,msg", MORE_BAD_CIRCUITS)
+def test_more_circuit_errors(text, msg):
+    with pytest.raises(RuntimeError) as e:
+        qflex.load_circuit(text.split("\n"))
+    assert msg in str(e.value)
+
+
+def test_circuit_grammar_details():
+    # spaces and tabs around parameters, a comment-only line numbered but
+    # skipped, "3.0" as a valid integer, fsim counted twice in the depth
+    n, depth, gates = qflex.load_circuit(
+        ["4", "# layer 0", "0 h 0", "1  rz ( 0.5 )\t2", "1 fsim( 0.25 ,0.1 ) 0 1",
+         "3.0 cz 2 3", "4 t 1 # trailing"])
+    assert n == 4 and depth == 2
+    assert gates == [(0, "h", [0], []), (1, "rz", [2], [0.5]), (1, "fsim", [0, 1], [0.25, 0.1]),
+                     (3, "cz", [2, 3], []), (4, "t", [1], [])]
+    # the qubit count must be on the first physical line: after a leading
+    # comment line the count is read as a gate line
+    with pytest.raises(RuntimeError, match=r"\[2: 4\] Gate must be specified as"):
+        qflex.load_circuit(["# comment", "4", "0 h 0"])
+
+
+def test_missing_files_raise(tmp_path):
+    with pytest.raises(RuntimeError, match="Cannot open circuit file"):
+        qflex.evaluate({"circuit_filename": str(tmp_path / "none"), "grid": ["1"],
+                        "ordering": ["expand a 0"]})
+    with pytest.raises(RuntimeError, match="Cannot open grid file: "):
+        qflex.evaluate({"grid_filename": str(tmp_path / "none")})
+
+
+def test_grid_details():
+    # rows count non-comment lines (even rows that end up empty) for the
+    # error text; a line without 0/1 characters is no row
+    I, J, off = qflex.load_grid(["# c", "1 0", "abc", "0 1 # 2"])
+    assert (I, J, off) == (2, 2, [[0, 1], [1, 0]])
+    with pytest.raises(RuntimeError, match="Grid size at line 3 is inconsistent with a width of 1 "
+                                           "instead of 2"):
+        qflex.load_grid(["11", "x", "1"])
+
+
+@pytest.mark.parametrize("line,msg", [
+    ("cut (a) 1 2", "Cut values must be comma-separated ints, e.g. (0,1,3)."),
+    ("cut 0,1 1 2", "Cut values must be comma-separated ints, e.g. (0,1,3)."),
+    ("cut () 4 1", "Index 1 must specify an active qubit."),
+    ("cut () 1 4", "Index 2 must specify an active qubit."),
+    ("cut () 4", "Index 1 must specify an active qubit."),
+    ("expand a", "Index -1 must be within grid boundaries."),
+    ("cut ()", "Index -1 must be within grid boundaries."),
+    ("cut () 1 6", "Index 6 must be within grid boundaries."),
+])
+def test_more_ordering_errors(line, msg):
+    with pytest.raises(RuntimeError) as e:
+        qflex.parse_ordering([line], ["11", "11", "01"])
+    assert f"Parsing failed on line: '{line}' with error: {msg}" in str(e.value)
+
+
+def test_ordering_grammar_details():
+    grid = ["11", "11", "01"]
+    # spaces inside the values are dropped by the line rules: "(1 2)" is (12)
+    assert qflex.parse_ordering(["cut (1 2) 0 1"], grid) == [("cut", [[0, 0], [0, 1]], [12])]
+    # empty pieces read as 0; a terminal cut has one qubit
+    assert qflex.parse_ordering(["cut (1,,2) 5"], grid) == [("cut", [[2, 1]], [1, 0, 2])]
+    # a merge with one name merges into the unnamed patch
+    assert qflex.parse_ordering(["expand a 0", "merge a"], grid) == \
+        [("expand", "a", [0, 0]), ("merge", "a", "")]
+    # comment and blank lines are dropped by the loader
+    assert qflex.parse_ordering(["# x", "", "expand b 3 # y"], grid) == [("expand", "b", [1, 1])]
+
+
+def test_validate_ordering_collects_every_problem():
+    ops = [("expand", "a", [0, 0]), ("merge", "a", "b"), ("expand", "a", [0, 1]),
+           ("expand", "c", [0, 0]), ("merge", "a", "b")]
+    with pytest.raises(RuntimeError) as e:
+        qflex.validate_ordering(ops)
+    msg = str(e.value)
+    assert msg.count("\n") == 3
+    assert "Tensor at (0,1) is added to non-empty patch a after a cut." in msg
+    assert "Tensor (0,0) is contracted multiple times." in msg
+    assert "Patch a is merged multiple times." in msg
+    # a well-formed ordering passes
+    qflex.validate_ordering([("cut", [[0, 0], [0, 1]], []), ("expand", "a", [0, 0]),
+                             ("expand", "b", [0, 1]), ("merge", "a", "b")])
+
+
+def test_simulate_input_type_errors(capfd):
+    for opts, msg in [({"grid": "11"}, "'grid' must be a list of strings."),
+                      ({"grid": ["1", 1]}, "grid must be a list of strings."),
+                      ({"circuit_filename": 3}, "'circuit_filename' must be a string."),
+                      ({"initial_states": 5}, "initial_states must be a list of strings."),
+                      ({"final_states": ["0", 1]}, "states must be strings."),
+                      ({"final_state": 0}, "states must be strings."),
+                      ({"memory_limit": 1.5}, "'memory_limit' must be valid.")]:
+        assert qflex.simulate(opts) == []
+        assert msg in capfd.readouterr().err, opts
+
+
 # ---------------------------------------------------------------- cost-model candidates
 
 @pytest.mark.parametrize("I,J", [(4, 4), (5, 6), (6, 6), (7, 7)])
