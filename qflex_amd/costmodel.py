@@ -8,10 +8,10 @@ The plain (row snake) ordering is the fallback.
 * candidates    -- seeded orderings of an I x J grid: one patch swept
                    along rows or columns (snake or raster, from any corner),
                    or two patches split along a row/column, each swept, then
-                   merged, with 0-2 bond cuts across the split.  The nearest
-                   analogue in the reference is the greedy
-                   qflexcirq/ordering/order_circuit_simulation.py:335-435
-                   (bond costs :87-106), which needs cirq.
+                   merged, with 0-2 bond cuts across the split; plus the
+                   greedy bond-contraction orderings with 0-2 greedy cuts
+                   (qflex_amd.greedy, the search of the reference's
+                   qflexcirq/ordering/order_circuit_simulation.py:261-421).
 * features      -- 64 floats from the engine's symbolic plan (qflex.plan: the
                    same pass that plans device memory): log2 flops, peak
                    tensor rank and size, per-step GEMM shapes pooled.
@@ -235,6 +235,19 @@ def candidate_set(I, J, rng, n):
     return out
 
 
+def greedy_candidates(circ_text):
+    """Orderings from the greedy bond-contraction search (qflex_amd.greedy):
+    no cut, and 1 or 2 greedy cuts chosen by the cost of one slice (the
+    reference's criterion) or of all slices; duplicates dropped."""
+    from . import greedy
+    out = []
+    for max_cuts, per_slice in ((0, True), (1, True), (2, True), (1, False), (2, False)):
+        text, _ = greedy.greedy_ordering(circ_text, max_cuts=max_cuts, per_slice=per_slice)
+        if text not in out:
+            out.append(text)
+    return out
+
+
 def feasible(plan, limit_elems):
     return all(s["out_size"] <= limit_elems for s in plan["steps"])
 
@@ -263,7 +276,7 @@ def run(args):
     for c in range(args.train_circuits):
         I, J, depth = shapes[c % len(shapes)]
         circ = circuits.random_circuit(I, J, depth, 1000 + c)
-        for order in candidate_set(I, J, rng, args.candidates):
+        for order in candidate_set(I, J, rng, args.candidates) + greedy_candidates(circ):
             opts = options(circ, order, I, J, args.precision)
             plan = qflex.plan(opts)
             if not ok(plan):
@@ -281,6 +294,7 @@ def run(args):
         I, J, depth = shapes[c % len(shapes)]
         circ = circuits.random_circuit(I, J, depth, 500000 + c)
         cands = candidate_set(I, J, rng, args.heldout_candidates)
+        cands += [g for g in greedy_candidates(circ) if g not in cands]
         feats, usable, flops = [], [], []
         for order in cands:
             plan = qflex.plan(options(circ, order, I, J, args.precision))
