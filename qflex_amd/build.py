@@ -85,6 +85,25 @@ def build_host():
     return out
 
 
+def install_into_package(package_dir):
+    """Copies the built module qflex (and lib/libqfx.so, which it loads from
+    lib/ beside itself) into `package_dir`, e.g. an installed qflexcirq
+    package, so that the reference's ``from qflexcirq import qflex``
+    (qflexcirq/interface/qflex_simulator.py:5) imports this engine unchanged.
+    Returns the module's path there."""
+    import glob
+    import shutil
+    mods = glob.glob(os.path.join(PKG, "qflex" + sysconfig.get_config_var("EXT_SUFFIX")))
+    lib = os.path.join(LIBDIR, "libqfx.so")
+    if not mods or not os.path.exists(lib):
+        raise RuntimeError("build first: qflex module or libqfx.so missing")
+    os.makedirs(os.path.join(package_dir, "lib"), exist_ok=True)
+    shutil.copy2(lib, os.path.join(package_dir, "lib", "libqfx.so"))
+    dst = os.path.join(package_dir, os.path.basename(mods[0]))
+    shutil.copy2(mods[0], dst)
+    return dst
+
+
 def build_all():
     lib = build_capi()
     mod = build_host() if os.path.exists(os.path.join(HOST, PYBIND_SOURCE)) else None
