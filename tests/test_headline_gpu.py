@@ -29,14 +29,22 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 BAR = {"fp32": 1e-5, "split_fp16": 1e-5, "split_bf16x3": 1e-5, "split_bf16": 5e-5}
 
 
-def load(slices):
+def load(slices, fixture_ordering=False):
+    """The circuit, an ordering restricted to the fixture's cut values, and
+    the fixture.  The amplitudes do not depend on the contraction order, so
+    the current corner-inward ordering and the one the fixture was written
+    with (recorded in it) must both reproduce them."""
     ref = json.load(open(os.path.join(GOLDEN, f"headline_7x7_d32_seed0_slices{slices}.json")))
     c = ref["circuit"]
     assert (c["I"], c["J"], c["depth"], c["seed"]) == (7, 7, 32, 0)
     circ = circuits.random_circuit(7, 7, 32, 0)
     cut_values = tuple(ref["cut_values"]) if ref.get("cut_values") else (None, None)
+    if fixture_ordering:
+        return circ, "\n".join(ref["ordering"]) + "\n", ref
     order = circuits.corner_inward_7x7(cut_values)
-    assert order.splitlines() == ref["ordering"]
+    # same cuts (and cut values) as the fixture's ordering
+    cuts = lambda lines: [l for l in lines if l.startswith("cut")]
+    assert cuts(order.splitlines()) == cuts(ref["ordering"])
     return circ, order, ref
 
 
@@ -65,6 +73,17 @@ def test_headline_subjob_vs_complex128(slices, precision):
     assert err <= BAR[precision], err
     st = qflex.last_stats()
     assert st["slices"] == slices
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split_fp16"])
+def test_headline_fixture_ordering_vs_complex128(precision):
+    """The ordering recorded in the one-slice fixture (a different tail
+    from the current corner-inward one) gives the same amplitudes."""
+    circ, order, ref = load(1, fixture_ordering=True)
+    got = run(circ, order, precision)
+    err = compare(got, ref)
+    print(f"7x7 d32 1 slice, fixture ordering, {precision}: relerr {err:.3e}")
+    assert err <= BAR[precision], err
 
 
 @pytest.mark.parametrize("precision", ["split_fp16", "fp32"])
