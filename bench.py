@@ -31,6 +31,16 @@ METRIC = "amplitudes/s, 7x7 depth-32 grid: plain FP32 vs split-BF16 at rel.errâ‰
 PEAK_TFLOPS = {"fp32": 157.3, "split_fp16": 2500.0, "split_bf16x3": 2500.0, "split_bf16": 2500.0,
                "fp8": 5000.0}
 HBM_GBPS = 8000.0
+# The metric names split-BF16 (hi+mid, 3 products); that form misses the
+# 1e-5 bar on this workload (~1.9e-5), so the headline may be another split.
+VARIANT_NOTE = {
+    "split_fp16": "deviation from the metric's split-BF16: the same 3-product hi+lo split "
+                  "with FP16 planes of power-of-two-scaled operands (22 significant bits); "
+                  "the literal split_bf16 is reported under variants",
+    "split_bf16x3": "deviation from the metric's split-BF16: hi+mid+lo BF16, 6 products",
+    "split_bf16": "the metric's split-BF16 form (hi+mid, 3 products)",
+    "fp32": "plain FP32 MFMA",
+}
 ERR_BAR = 1e-5
 
 
@@ -45,12 +55,16 @@ def workload(seed):
 
 
 AUTOTUNE = True  # tile autotuner on (the warmup step does the timing of new products)
+# Values of each bond cut contracted together (simulate's "slice_batch"): the
+# inner cut's 16 values as one batch, the outer cut one value at a time.
+SLICE_BATCH = [1, 16]
 
 
 def options(circ, order, grid, precision):
     return {"circuit": circ.splitlines(), "ordering": order.splitlines(),
             "grid": grid.splitlines(), "initial_state": "0" * 49, "final_state": "",
-            "memory_limit": "200GB", "precision": precision, "autotune": AUTOTUNE}
+            "memory_limit": "200GB", "precision": precision, "autotune": AUTOTUNE,
+            "slice_batch": SLICE_BATCH}
 
 
 def relerr(got, ref):
@@ -267,6 +281,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true",
                     help="fixed tile rule instead of the tile autotuner")
+    ap.add_argument("--slice-batch", default=",".join(str(x) for x in SLICE_BATCH),
+                    help="values of each bond cut contracted together, one per cut in ordering "
+                         "order ('1' = one slice at a time, the reference's loop)")
     ap.add_argument("--microbench", default="",
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
                          "with HIP events through the C-ABI, each --steps times per variant); "
@@ -279,8 +296,9 @@ def main():
                     help="first run one device copy of this many MiB (qfx_copy_kernel), the "
                          "known-size reference for HBM byte counters")
     args = ap.parse_args()
-    global AUTOTUNE
+    global AUTOTUNE, SLICE_BATCH
     AUTOTUNE = not args.no_autotune
+    SLICE_BATCH = [int(x) for x in args.slice_batch.split(",")]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -339,8 +357,12 @@ def main():
 
     ok = {k: v for k, v in variants.items()
           if v["relerr_vs_complex128"] is not None and v["relerr_vs_complex128"] <= ERR_BAR}
+    # the headline is the fastest variant that meets the error bar; when none
+    # does, value is null (the fastest variant still fills the rest)
     pick = max(ok or variants, key=lambda k: variants[k]["amplitudes_per_s"])
     best = variants[pick]
+    if not ok:
+        log(f"[bench] no variant meets the {ERR_BAR} error bar: value is null")
     # reorders: the variant that moves the most bytes through them (fp32,
     # which reorders every operand; the split variants read operands in their
     # stored order and keep only a few small reorders)
@@ -354,7 +376,7 @@ def main():
     achieved = g["flops"] / g["ms"] / 1e9
     out = {
         "metric": METRIC,
-        "value": best["amplitudes_per_s"],
+        "value": best["amplitudes_per_s"] if ok else None,
         "unit": "amplitudes/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -369,7 +391,9 @@ def main():
         "config": {"workload": "7x7 grid, depth 1+32+1, bond 16, corner-inward ordering, "
                                "2 cuts = 256 slices, 6-qubit open region = 64 amplitudes per step",
                    "variant": pick, "precision_bar": ERR_BAR, "autotune": AUTOTUNE,
-                   "passes_bar": pick in ok},
+                   "slice_batch": SLICE_BATCH,
+                   "passes_bar": pick in ok,
+                   "variant_note": VARIANT_NOTE.get(pick, "")},
         "variants": variants,
         "roofline": {"bound": "mfma", "kernel": "cgemm_" + pick,
                      "achieved": achieved, "peak": PEAK_TFLOPS[pick], "unit": "TFLOP/s",

@@ -236,6 +236,16 @@ int qfx_cdotu_c64(qfx_context* ctx, uint64_t n, const qfx_c64* x, const qfx_c64*
 int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n, qfx_c128* amps,
                         uint64_t offset);
 
+/* amps[offset + i] += (complex double) src[b][i] for b = 0 .. nbatch-1 (src is
+ * [nbatch][n]), one FP64 addition per b in increasing b: the same sums, in
+ * the same order, as nbatch calls of qfx_accumulate_c128.  For a contraction
+ * whose bond-cut values are processed as a batch (the cut index kept as a
+ * batch index of dimension nbatch); it replaces the per-slice accumulation
+ * (*amplitudes_)[output_index] += scalar of ContractionData::ContractGrid
+ * (src/contraction_utils.cpp:352) for nbatch slices at once. */
+int qfx_accumulate_batched_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n,
+                                uint64_t nbatch, qfx_c128* amps, uint64_t offset);
+
 /* ---- recorded sequences (HIP graphs) ------------------------------------- */
 /* Between qfx_graph_begin and qfx_graph_end the context's calls are recorded
  * (HIP stream capture) instead of run; qfx_graph_launch replays the whole

@@ -80,6 +80,7 @@ SIGNATURES = {
     "qfx_cgemv_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _vp, _vp, _vp]),
     "qfx_cdotu_c64": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
     "qfx_accumulate_c128": (_c.c_int, [_vp, _vp, _u64, _vp, _u64]),
+    "qfx_accumulate_batched_c128": (_c.c_int, [_vp, _vp, _u64, _u64, _vp, _u64]),
     "qfx_mlp_forward": (_c.c_int, [_vp, _u64, _vp, _vp, _vp]),
     "qfx_mlp_backward": (_c.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp]),
     "qfx_mlp_adam": (_c.c_int, [_vp, _vp, _vp, _vp, _vp, _c.c_int, _c.c_float, _c.c_float,
@@ -199,6 +200,12 @@ class Context:
     def copy(self, dst, src, nbytes):
         check(load().qfx_copy_d2d(self.handle, dst.ptr, src.ptr, nbytes))
 
+    def arena_stats(self):
+        """(bytes reserved from HIP, bytes handed out) of the context's arena."""
+        r, u = _c.c_size_t(), _c.c_size_t()
+        check(load().qfx_arena_stats(self.handle, _c.byref(r), _c.byref(u)))
+        return r.value, u.value
+
     def sync(self):
         check(load().qfx_synchronize(self.handle))
 
@@ -299,6 +306,10 @@ class Context:
 
     def accumulate(self, src, n, amps, offset=0):
         check(load().qfx_accumulate_c128(self.handle, src.ptr, n, amps.ptr, offset))
+
+    def accumulate_batched(self, src, n, nbatch, amps, offset=0):
+        check(load().qfx_accumulate_batched_c128(self.handle, src.ptr, n, nbatch, amps.ptr,
+                                                 offset))
 
     def profile(self, on=True):
         check(load().qfx_profile_enable(self.handle, 1 if on else 0))

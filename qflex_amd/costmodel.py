@@ -104,7 +104,7 @@ def features(plan, I, J, depth):
     m = np.array([s["m"] for s in st], dtype=np.float64)
     n = np.array([s["n"] for s in st], dtype=np.float64)
     k = np.array([s["k"] for s in st], dtype=np.float64)
-    mult = np.array([s["multiplicity"] for s in st], dtype=np.float64)
+    mult = np.array([s["multiplicity"] * s.get("batch", 1) for s in st], dtype=np.float64)
     out = np.array([s["out_size"] for s in st], dtype=np.float64)
     rank = np.array([s["out_rank"] for s in st], dtype=np.float64)
     flops = 8.0 * m * n * k * mult
@@ -253,7 +253,8 @@ def feasible(plan, limit_elems):
 
 
 def plan_flops(plan):
-    return sum(8.0 * s["m"] * s["n"] * s["k"] * s["multiplicity"] for s in plan["steps"])
+    return sum(8.0 * s["m"] * s["n"] * s["k"] * s.get("batch", 1) * s["multiplicity"]
+               for s in plan["steps"])
 
 
 def log(msg):

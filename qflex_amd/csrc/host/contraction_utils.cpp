@@ -235,6 +235,38 @@ void ValidateOrdering(const std::list<ContractionOperation>& ordering) {
 
 // ------------------------------------------------------------ planning
 
+std::unordered_map<std::string, std::size_t> bond_cut_batches(
+    const std::list<ContractionOperation>& ordering,
+    const std::vector<std::vector<Tensor>>& tensor_grid) {
+  std::unordered_map<std::string, std::size_t> out;
+  std::size_t ordinal = 0;
+  for (const auto& op : ordering) {
+    if (op.op_type != ContractionOperation::CUT || op.cut.tensors.size() != 2) continue;
+    const std::size_t want =
+        global::slice_batch.empty()
+            ? 1
+            : global::slice_batch[std::min(ordinal, global::slice_batch.size() - 1)];
+    ++ordinal;
+    std::string name;
+    try {
+      name = index_name(op.cut.tensors);
+    } catch (const std::string&) {
+      continue;  // reported by the planner and the interpreter
+    }
+    std::size_t count = op.cut.values.size();
+    if (count == 0) {
+      const auto& t = op.cut.tensors[0];
+      if (t.size() < 2 || t[0] >= tensor_grid.size() || t[1] >= tensor_grid[t[0]].size())
+        continue;
+      const auto& dims = tensor_grid[t[0]][t[1]].get_index_to_dimension();
+      auto it = dims.find(name);
+      count = it == dims.end() ? 1 : it->second;
+    }
+    out[name] = std::max<std::size_t>(1, std::min(want, count));
+  }
+  return out;
+}
+
 SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
                               const std::vector<std::vector<Tensor>>& tensor_grid) {
   SymbolicPlan plan;
@@ -253,11 +285,25 @@ SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
 
   // Bond cuts remove their index (and multiply the work after them by
   // their value count); terminal cuts keep the output index open with one
-  // entry per requested value.
+  // entry per requested value.  A bond cut taken a batch of values at a
+  // time keeps its index, of the batch's size, as a batch index: present in
+  // both operands it is a batch dimension of the product (never summed), and
+  // the work after the cut runs once per batch.
+  const auto batches = bond_cut_batches(ordering, tensor_grid);
+  std::vector<std::string> batch;
   std::unordered_map<std::string, std::vector<std::string>> patch_indices;
   std::size_t multiplicity = 1;
-  auto sym_diff = [](const std::vector<std::string>& a, const std::vector<std::string>& b) {
-    return _vector_subtraction(_vector_union(a, b), _vector_intersection(a, b));
+  auto batch_common = [&](const std::vector<std::string>& a, const std::vector<std::string>& b) {
+    std::vector<std::string> out;
+    for (const auto& x : _vector_intersection(a, b))
+      if (std::find(batch.begin(), batch.end(), x) != batch.end()) out.push_back(x);
+    return out;
+  };
+  // indices of the product of a and b: batch indices of both, then the
+  // free indices of each
+  auto sym_diff = [&](const std::vector<std::string>& a, const std::vector<std::string>& b) {
+    return _vector_union(batch_common(a, b),
+                         _vector_subtraction(_vector_union(a, b), _vector_intersection(a, b)));
   };
   auto size_of = [&](const std::vector<std::string>& idx) {
     std::size_t s = 1;
@@ -270,9 +316,11 @@ SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
     if (a.empty() || b.empty()) return;
     ContractionStep st;
     st.op = op;
+    const auto bc = batch_common(a, b);
     st.m = size_of(_vector_subtraction(a, b));
-    st.k = size_of(_vector_intersection(a, b));
+    st.k = size_of(_vector_subtraction(_vector_intersection(a, b), bc));
     st.n = size_of(_vector_subtraction(b, a));
+    st.batch = size_of(bc);
     st.multiplicity = multiplicity;
     const auto c = sym_diff(a, b);
     st.out_rank = c.size();
@@ -295,14 +343,16 @@ SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
           const std::string cut_index = index_name(op.cut.tensors);
           const std::size_t dim = index_dimension.count(cut_index) ? index_dimension[cut_index] : 1;
           const std::size_t count = op.cut.values.empty() ? dim : op.cut.values.size();
+          const std::size_t group =
+              op.cut.tensors.size() > 1 && batches.count(cut_index) ? batches.at(cut_index) : 1;
           int side = 0;
           for (const auto& tensor : op.cut.tensors) {
             auto& indices = grid_indices.at(tensor[0]).at(tensor[1]);
             const std::size_t full = tensor_grid[tensor[0]][tensor[1]].size();
             if (op.cut.tensors.size() > 1) {
-              indices = _vector_subtraction(indices, {cut_index});
+              if (group == 1) indices = _vector_subtraction(indices, {cut_index});
               plan.side_buffers[ContractionData::cut_copy_name(op.cut.tensors, side)] =
-                  full / std::max<std::size_t>(dim, 1);
+                  full / std::max<std::size_t>(dim, 1) * group;
             } else {
               index_dimension[cut_index] = count;
               plan.side_buffers[ContractionData::cut_copy_name(op.cut.tensors, side)] =
@@ -310,7 +360,13 @@ SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
             }
             ++side;
           }
-          if (op.cut.tensors.size() > 1) multiplicity *= count;
+          if (op.cut.tensors.size() > 1) {
+            multiplicity *= (count + group - 1) / group;
+            if (group > 1) {
+              index_dimension[cut_index] = group;
+              batch.push_back(cut_index);
+            }
+          }
         } catch (const std::string& err_msg) {
           throw ERROR_MSG("Failed during CUT. Error:\n\t[", err_msg, "]");
         }
@@ -491,7 +547,138 @@ void ContractionData::grid_operand_maxima(const View& v, const float** out, int*
   *count = it->second.second;
 }
 
+bool ContractionData::is_batch(const std::string& index) const {
+  return std::find(batch_.begin(), batch_.end(), index) != batch_.end();
+}
+
+// C[batch][left][right] = sum_common A[batch, left, common] B[batch, common,
+// right] for every value of the batch indices both operands carry: one
+// product per batch value, each read in the operands' stored orders where a
+// kernel supports it (else both operands are reordered to (batch, left,
+// common) / (batch, common, right) once and the products read contiguous
+// blocks).  The split-FP16 scale of every product comes from the maxima of
+// the whole operand.
+ContractionData::View ContractionData::contract_batched(const View& A, const View& B, s_type* out,
+                                                        const std::vector<std::string>& bc) {
+  std::unordered_map<std::string, std::size_t> adim, bdim;
+  for (std::size_t i = 0; i < A.indices.size(); ++i) adim[A.indices[i]] = A.dims[i];
+  for (std::size_t i = 0; i < B.indices.size(); ++i) bdim[B.indices[i]] = B.dims[i];
+  const std::vector<std::string> left = _vector_subtraction(A.indices, B.indices);
+  const std::vector<std::string> right = _vector_subtraction(B.indices, A.indices);
+  const std::vector<std::string> common =
+      _vector_subtraction(_vector_intersection(A.indices, B.indices), bc);
+  std::size_t m = 1, n = 1, k = 1, nb = 1;
+  for (const auto& x : left) m *= adim[x];
+  for (const auto& x : right) n *= bdim[x];
+  for (const auto& x : common) {
+    if (adim[x] != bdim[x])
+      throw ERROR_MSG("Common indices must have matching dimensions, but at index: ", x,
+                      ", the dimensions are A: ", adim[x], ", B: ", bdim[x], ".");
+    k *= adim[x];
+  }
+  for (const auto& x : bc) {
+    if (adim[x] != bdim[x])
+      throw ERROR_MSG("Batch index ", x, " has dimension ", adim[x], " in A and ", bdim[x],
+                      " in B.");
+    nb *= adim[x];
+  }
+  // element strides of the batch indices in A and B
+  auto strides = [](const View& v, const std::vector<std::string>& names) {
+    std::vector<std::size_t> out;
+    for (const auto& x : names) {
+      std::size_t st = 1;
+      const std::size_t p = std::size_t(std::find(v.indices.begin(), v.indices.end(), x) -
+                                        v.indices.begin());
+      for (std::size_t a = p + 1; a < v.dims.size(); ++a) st *= v.dims[a];
+      out.push_back(st);
+    }
+    return out;
+  };
+  const auto sa = strides(A, bc), sb = strides(B, bc);
+  std::vector<std::size_t> bdims;
+  for (const auto& x : bc) bdims.push_back(adim[x]);
+  // offsets of batch value b (row-major over bc, first slowest)
+  auto offsets = [&](std::size_t b, std::size_t* oa, std::size_t* ob) {
+    *oa = *ob = 0;
+    for (std::size_t j = bc.size(); j-- > 0;) {
+      const std::size_t d = b % bdims[j];
+      b /= bdims[j];
+      *oa += d * sa[j];
+      *ob += d * sb[j];
+    }
+  };
+  qfx_scale_io io{};
+  io.a_max = A.amax;
+  io.a_max_n = A.amax_n;
+  io.b_max = B.amax;
+  io.b_max_n = B.amax_n;
+  if (global::precision == QFX_PREC_SPLIT_FP16) {
+    if (batch_amax_.empty())
+      for (int i = 0; i < 2; ++i)
+        batch_amax_.emplace_back(std::vector<std::string>{""}, std::vector<std::size_t>{128});
+    auto whole = [&](const View& v, int slot, const float** arr, int* cnt) {
+      if (*cnt > 0) return;
+      float* dst = reinterpret_cast<float*>(batch_amax_[slot].data());
+      qfx_check(qfx_amax_partials_c64(device(), reinterpret_cast<const qfx_c64*>(v.ptr), v.size(),
+                                      dst, 256, cnt),
+                "qfx_amax_partials_c64");
+      *arr = dst;
+    };
+    whole(A, 0, &io.a_max, &io.a_max_n);
+    whole(B, 1, &io.b_max, &io.b_max_n);
+  }
+  std::size_t done = 0;
+  if (!left.empty() && !right.empty() && global::mapped_operands) {
+    for (; done < nb; ++done) {
+      std::size_t oa, ob;
+      offsets(done, &oa, &ob);
+      qfx_scale_io one = io;  // no output maxima: each product writes a part of C
+      if (!try_mapped(A.ptr + oa, A.indices, A.dims, B.ptr + ob, B.indices, B.dims, left, common,
+                      right, out + done * m * n, &one))
+        break;
+    }
+  }
+  if (done < nb) {
+    const s_type* pa = A.ptr;
+    const s_type* pb = B.ptr;
+    const auto a_order = _vector_union(bc, _vector_union(left, common));
+    const auto b_order = _vector_union(bc, _vector_union(common, right));
+    if (a_order != A.indices) {
+      s_type* buf = get_scratch(kGeneralSpace).data();
+      device_reorder(A.ptr, buf, A.indices, A.dims, a_order);
+      pa = buf;
+      stats_.reorders++;
+      stats_.reorder_elements += double(A.size());
+    }
+    if (b_order != B.indices) {
+      s_type* buf = get_scratch(kGeneralSpaceB).data();
+      device_reorder(B.ptr, buf, B.indices, B.dims, b_order);
+      pb = buf;
+      stats_.reorders++;
+      stats_.reorder_elements += double(B.size());
+    }
+    for (; done < nb; ++done)
+      device_multiply(pa + done * m * k, pb + done * k * n, out + done * m * n, m, n, k,
+                      !left.empty(), !right.empty());
+  }
+  stats_.multiplies++;
+  stats_.flops += 8.0 * double(m) * double(n) * double(k) * double(nb);
+  View C;
+  C.ptr = out;
+  C.indices = _vector_union(bc, _vector_union(left, right));
+  for (const auto& x : bc) C.dims.push_back(adim[x]);
+  for (const auto& x : left) C.dims.push_back(adim[x]);
+  for (const auto& x : right) C.dims.push_back(bdim[x]);
+  return C;
+}
+
 ContractionData::View ContractionData::contract_pair(const View& A, const View& B, s_type* out) {
+  if (!batch_.empty()) {
+    std::vector<std::string> bc;
+    for (const auto& x : _vector_intersection(A.indices, B.indices))
+      if (is_batch(x)) bc.push_back(x);
+    if (!bc.empty()) return contract_batched(A, B, out, bc);
+  }
   std::vector<std::string> left = _vector_subtraction(A.indices, B.indices);
   std::vector<std::string> right = _vector_subtraction(B.indices, A.indices);
   std::vector<std::string> common = _vector_intersection(A.indices, B.indices);
@@ -612,7 +799,9 @@ ContractionData::View ContractionData::project_view(const View& v, const std::st
   } else {
     out.dims[p] = picks.size();
   }
-  if (pre == 1 && picks.size() == 1) {
+  bool consecutive = true;
+  for (std::size_t j = 1; j < picks.size(); ++j) consecutive = consecutive && picks[j] == picks[0] + j;
+  if (pre == 1 && consecutive && !picks.empty()) {
     out.ptr = v.ptr + picks[0] * post;  // contiguous slice: zero copy
     return out;
   }
@@ -635,7 +824,16 @@ void ContractionData::emit_output(const View& output, std::size_t output_index,
     open_names.push_back(o.index);
     P *= o.count;
   }
-  if (!_vector_subtraction(output.indices, open_names).empty() || output.size() != P) {
+  // batched cut indices still open: one amplitude vector per batch value
+  std::vector<std::string> batch_names;
+  std::size_t nb = 1;
+  for (std::size_t i = 0; i < output.indices.size(); ++i)
+    if (is_batch(output.indices[i])) {
+      batch_names.push_back(output.indices[i]);
+      nb *= output.dims[i];
+    }
+  if (!_vector_subtraction(_vector_subtraction(output.indices, open_names), batch_names).empty() ||
+      output.size() != P * nb) {
     std::string desc = "Tensor of rank " + std::to_string(output.indices.size()) + ": ";
     for (std::size_t i = 0; i < output.indices.size(); ++i)
       desc += output.indices[i] + " -> " + std::to_string(output.dims[i]) +
@@ -647,17 +845,23 @@ void ContractionData::emit_output(const View& output, std::size_t output_index,
     throw ERROR_MSG("Output index ", offset + P - 1, " is outside the amplitude vector of size ",
                     amplitudes_->size(), ".");
   const s_type* src = output.ptr;
-  if (open_names != output.indices) {
+  const std::vector<std::string> order = _vector_union(batch_names, open_names);
+  if (order != output.indices) {
     s_type* buf = get_scratch(kGeneralSpaceB).data();
     if (buf == output.ptr) buf = get_scratch(kGeneralSpace).data();
-    device_reorder(output.ptr, buf, output.indices, output.dims, open_names);
+    device_reorder(output.ptr, buf, output.indices, output.dims, order);
     src = buf;
   }
-  qfx_check(qfx_accumulate_c128(device(), reinterpret_cast<const qfx_c64*>(src), P,
-                                reinterpret_cast<qfx_c128*>(get_scratch(kAmplitudeSpace).data()),
-                                offset),
-            "qfx_accumulate_c128");
-  stats_.slices++;
+  qfx_c128* amps = reinterpret_cast<qfx_c128*>(get_scratch(kAmplitudeSpace).data());
+  if (nb == 1)
+    qfx_check(qfx_accumulate_c128(device(), reinterpret_cast<const qfx_c64*>(src), P, amps,
+                                  offset),
+              "qfx_accumulate_c128");
+  else
+    qfx_check(qfx_accumulate_batched_c128(device(), reinterpret_cast<const qfx_c64*>(src), P, nb,
+                                          amps, offset),
+              "qfx_accumulate_batched_c128");
+  stats_.slices += nb;
 }
 
 void ContractionData::run(std::list<ContractionOperation>& ordering, std::size_t output_index,
@@ -710,16 +914,44 @@ void ContractionData::run(std::list<ContractionOperation>& ordering, std::size_t
           const View saved_b = vb;
           s_type* buf_a = get_scratch(cut_copy_name(op.cut.tensors, 0)).data();
           s_type* buf_b = get_scratch(cut_copy_name(op.cut.tensors, 1)).data();
-          for (std::size_t val : values) {
-            try {
-              grid_views_[pa[0]][pa[1]] = project_view(saved_a, index, val, nullptr, buf_a);
-              grid_views_[pb[0]][pb[1]] = project_view(saved_b, index, val, nullptr, buf_b);
-            } catch (const std::string& err_msg) {
-              throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+          auto batch_it = cut_batches_.find(index);
+          const std::size_t group = batch_it == cut_batches_.end() ? 1 : batch_it->second;
+          if (group <= 1) {
+            for (std::size_t val : values) {
+              try {
+                grid_views_[pa[0]][pa[1]] = project_view(saved_a, index, val, nullptr, buf_a);
+                grid_views_[pb[0]][pb[1]] = project_view(saved_b, index, val, nullptr, buf_b);
+              } catch (const std::string& err_msg) {
+                throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+              }
+              std::list<ContractionOperation> rest(ordering);
+              auto active = active_patches;
+              run(rest, output_index, active, open_outputs);
             }
-            std::list<ContractionOperation> rest(ordering);
-            auto active = active_patches;
-            run(rest, output_index, active, open_outputs);
+          } else {
+            // A batch of values at a time: both tensors keep the cut index,
+            // restricted to the batch's values, as a batch index.
+            batch_.push_back(index);
+            for (std::size_t first = 0; first < values.size(); first += group) {
+              const std::vector<std::size_t> picks(
+                  values.begin() + first, values.begin() + std::min(first + group, values.size()));
+              try {
+                grid_views_[pa[0]][pa[1]] = project_view(saved_a, index, 0, &picks, buf_a);
+                grid_views_[pb[0]][pb[1]] = project_view(saved_b, index, 0, &picks, buf_b);
+              } catch (const std::string& err_msg) {
+                batch_.pop_back();
+                throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+              }
+              std::list<ContractionOperation> rest(ordering);
+              auto active = active_patches;
+              try {
+                run(rest, output_index, active, open_outputs);
+              } catch (...) {
+                batch_.pop_back();
+                throw;
+              }
+            }
+            batch_.pop_back();
           }
           grid_views_[pa[0]][pa[1]] = saved_a;
           grid_views_[pb[0]][pb[1]] = saved_b;
@@ -790,6 +1022,8 @@ void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
       }
     }
     patches_.clear();
+    cut_batches_ = bond_cut_batches(ordering, *tensor_grid_);
+    batch_.clear();
     grid_amax_.clear();
     grid_amax_store_.clear();
     grid_ranges_.clear();

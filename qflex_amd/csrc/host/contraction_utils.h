@@ -102,6 +102,10 @@ struct ContractionStats {
 struct ContractionStep {
   int op = 0;  // ContractionOperation::EXPAND or MERGE
   std::size_t m = 1, n = 1, k = 1;
+  // independent products of this shape per execution: the product of the
+  // dimensions of the batched cut indices both operands carry (1 without
+  // batched cuts)
+  std::size_t batch = 1;
   std::size_t multiplicity = 1;
   std::size_t out_rank = 0, out_size = 1;
 };
@@ -116,6 +120,12 @@ struct SymbolicPlan {
 };
 SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
                               const std::vector<std::vector<Tensor>>& tensor_grid);
+
+// Batch size of every bond cut of `ordering` under global::slice_batch, by
+// cut index name (at most the cut's value count; 1 = one value at a time).
+std::unordered_map<std::string, std::size_t> bond_cut_batches(
+    const std::list<ContractionOperation>& ordering,
+    const std::vector<std::vector<Tensor>>& tensor_grid);
 
 class ContractionData {
  public:
@@ -186,6 +196,16 @@ class ContractionData {
   std::unordered_map<std::string, View> patches_;
   int depth_ = 0;
   ContractionStats stats_;
+  // batch size of each bond cut (by index name), and the cut indices whose
+  // current values are a batch (kept open, never summed by a GEMM)
+  std::unordered_map<std::string, std::size_t> cut_batches_;
+  std::vector<std::string> batch_;
+  // device arrays for the split-FP16 partial maxima of whole batched
+  // operands (A, B)
+  std::vector<Tensor> batch_amax_;
+  bool is_batch(const std::string& index) const;
+  View contract_batched(const View& A, const View& B, s_type* out,
+                        const std::vector<std::string>& batch_common);
 
   void run(std::list<ContractionOperation>& ordering, std::size_t output_index,
            std::unordered_map<std::string, bool>& active_patches,

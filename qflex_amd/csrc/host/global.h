@@ -3,6 +3,7 @@
 #define QFLEX_AMD_GLOBAL_H_
 
 #include <cstddef>
+#include <vector>
 
 namespace qflex::global {
 
@@ -21,6 +22,14 @@ inline int precision = 0;
 // (common, right) order are read in their stored order by the mapped cGEMM
 // (no reorder) wherever a kernel supports the form.
 inline bool mapped_operands = true;
+
+// Bond cuts whose values are contracted a batch at a time: entry i is the
+// batch size of the i-th bond cut of the ordering (the last entry also
+// applies to later cuts).  Empty, or 1, is the reference's slice loop (one
+// value at a time).  A batch keeps the cut index, restricted to the batch's
+// values, as a batch index: it is never summed inside a GEMM, and the
+// amplitudes of the batch's slices are summed in FP64 as one per slice.
+inline std::vector<std::size_t> slice_batch;
 
 }  // namespace qflex::global
 

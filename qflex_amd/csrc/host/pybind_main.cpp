@@ -4,10 +4,13 @@
 // (src/pybind_main.cpp:75-138, src/pybind_main.h:19-23): the same keys
 // (grid/circuit/ordering as lists of lines or *_filename, initial_state(s),
 // final_state(s), memory_limit), the same single-state restriction, and
-// errors printed to stderr with [] returned.  Two extra optional keys:
+// errors printed to stderr with [] returned.  Extra optional keys:
 // "precision" ("fp32" | "split_fp16" | "split_bf16x3" | "split_bf16" |
 // "fp32_exact"), "verbose" (int) and "autotune" (bool: time the GEMM tiles
-// on first use and keep the fastest, for this and later calls).
+// on first use and keep the fastest, for this and later calls) and
+// "slice_batch" (int, or one int per bond cut in ordering order: the number
+// of cut values contracted together, the cut index kept as a batch index;
+// default 1, the reference's one slice at a time).
 //
 // The remaining functions expose the host's Tensor / contraction API to the
 // tests; they raise RuntimeError with the reference's message texts.
@@ -117,6 +120,19 @@ void load_input(const py::dict& options, qflex::QflexInput& input) {
   if (options.contains("precision"))
     qflex::global::precision = parse_precision(options["precision"].cast<std::string>());
   if (options.contains("verbose")) qflex::global::verbose = options["verbose"].cast<int>();
+  qflex::global::slice_batch.clear();
+  if (options.contains("slice_batch")) {
+    const auto& sb = options["slice_batch"];
+    std::vector<long> sizes;
+    if (py::isinstance<py::int_>(sb))
+      sizes.push_back(sb.cast<long>());
+    else
+      sizes = sb.cast<std::vector<long>>();
+    for (long v : sizes) {
+      if (v < 1) throw ERROR_MSG("'slice_batch' entries must be positive.");
+      qflex::global::slice_batch.push_back(std::size_t(v));
+    }
+  }
   if (options.contains("autotune"))
     qflex::qfx_check(qfx_autotune_enable(qflex::device(), options["autotune"].cast<bool>() ? 1 : 0),
                      "qfx_autotune_enable");
@@ -152,6 +168,7 @@ py::dict plan(const py::dict& options) {
     d["n"] = st.n;
     d["k"] = st.k;
     d["multiplicity"] = st.multiplicity;
+    d["batch"] = st.batch;
     d["out_rank"] = st.out_rank;
     d["out_size"] = st.out_size;
     steps.append(d);

@@ -663,19 +663,39 @@ __device__ __forceinline__ void split_big_tile(
   // A, AM 2:   kk = f & 7, row = 2*(f >> 3) (complex rows row, row+1)
   // B: kk = g & 7 (consecutive lanes take consecutive k, so the LDS stores
   //    are conflict-free), nq = g >> 3 (complex n = 2nq, 2nq+1)
+  // Each operand is read through a buffer resource based at the block's
+  // first row (A) or column (B) and at the stage's first k, so only one
+  // stage's footprint in the block, not the whole operand, has to fit the
+  // 32-bit offsets.  For a digit map, offset(i0 + d) = offset(i0) +
+  // offset(d) when i0 is a multiple of the power-of-two tile edge (or of
+  // the stage depth) and d is below it.
+  uint64_t a_base, b_base, a_span, b_span;  // in elements
+  if constexpr (AM == 0) {
+    a_base = uint64_t(m0) * uint64_t(K);
+    b_base = uint64_t(n0);
+    a_span = uint64_t(kBM - 1) * uint64_t(K) + kBBK;
+    b_span = uint64_t(kBBK - 1) * uint64_t(N) + kBN;
+  } else {
+    a_base = map_offset(a_rows, uint64_t(m0));
+    b_base = map_offset(b_cols, uint64_t(n0));
+    a_span = map_offset(a_rows, uint64_t(min(kBM, M) - 1)) +
+             map_offset(a_cols, uint64_t(min(kBBK, K) - 1)) + 1;
+    b_span = map_offset(b_cols, uint64_t(min(kBN, N) - 1)) +
+             map_offset(b_rows, uint64_t(min(kBBK, K) - 1)) + 1;
+  }
   uint64_t a_off[kAH], a_coff = 0, b_off[kBH], b_roff = 0;
   if constexpr (AM != 0) {
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + NTH * h;
       const int row = AM == 1 ? (f >> 2) : 2 * (f >> 3);
-      a_off[h] = map_offset(a_rows, uint64_t(m0 + row));
+      a_off[h] = map_offset(a_rows, uint64_t(row));
     }
     a_coff = map_offset(a_cols, AM == 1 ? uint64_t(2 * (tid & 3)) : uint64_t(tid & 7));
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
-      b_off[h] = map_offset(b_cols, uint64_t(n0 + 2 * (g >> 3)));
+      b_off[h] = map_offset(b_cols, uint64_t(2 * (g >> 3)));
     }
     b_roff = map_offset(b_rows, uint64_t(tid & 7));
   }
@@ -693,11 +713,9 @@ __device__ __forceinline__ void split_big_tile(
   // Operands are read through buffer resources: an element outside the
   // tile's range gets an offset past the end of the buffer and reads as 0,
   // so there is no branch and no select near the loads (the waits for a
-  // stage's operands then never include the next stage's loads).
-  const __amdgpu_buffer_rsrc_t rsrc_a = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float2*>(A), 0, int(uint64_t(M) * K * sizeof(float2)), 0x00020000);
-  const __amdgpu_buffer_rsrc_t rsrc_b = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float2*>(B), 0, int(uint64_t(K) * N * sizeof(float2)), 0x00020000);
+  // stage's operands then never include the next stage's loads).  The host
+  // checks that both spans fit (big_operands_fit, big_mapped_spans_fit).
+  const int a_rec = int(a_span * sizeof(float2)), b_rec = int(b_span * sizeof(float2));
   constexpr unsigned kOutside = 0xfffffff0u;
   auto load16 = [](__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
     const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, int(byte_off), 0, 0);
@@ -706,11 +724,22 @@ __device__ __forceinline__ void split_big_tile(
   };
   auto load_tile = [&](int kt, float4* ra, float4* rb) {
     const int k0 = kt * kBBK;
-    unsigned a_kbase = 0, b_kbase = 0;
+    uint64_t a_stage, b_stage;  // the stage's first k, in elements from the block bases
     if constexpr (AM != 0) {
-      a_kbase = unsigned(ktab[2 * kt]);
-      b_kbase = unsigned(ktab[2 * kt + 1]);
+      // uniform values read from LDS: made scalar so the resources stay in SGPRs
+      const uint64_t ta = ktab[2 * kt], tb = ktab[2 * kt + 1];
+      a_stage = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(ta))))) |
+                (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(ta >> 32))))) << 32);
+      b_stage = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(tb))))) |
+                (uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(tb >> 32))))) << 32);
+    } else {
+      a_stage = uint64_t(k0);
+      b_stage = uint64_t(k0) * uint64_t(N);
     }
+    const __amdgpu_buffer_rsrc_t rsrc_a = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2*>(A + a_base + a_stage), 0, a_rec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsrc_b = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float2*>(B + b_base + b_stage), 0, b_rec, 0x00020000);
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + NTH * h;
@@ -719,12 +748,12 @@ __device__ __forceinline__ void split_big_tile(
       if constexpr (AM == 2) {
         gm = m0 + 2 * (f >> 3);
         gk = k0 + (f & 7);
-        off = unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
+        off = unsigned(a_off[h]) + unsigned(a_coff);
       } else {
         gm = m0 + (f >> 2);
         gk = k0 + 2 * (f & 3);
-        off = AM == 0 ? unsigned(gm) * unsigned(K) + unsigned(gk)
-                      : unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
+        off = AM == 0 ? unsigned(gm - m0) * unsigned(K) + unsigned(gk - k0)
+                      : unsigned(a_off[h]) + unsigned(a_coff);
       }
       ra[h] = load16(rsrc_a, (gm < M && gk < K) ? off * unsigned(sizeof(float2)) : kOutside);
     }
@@ -732,8 +761,8 @@ __device__ __forceinline__ void split_big_tile(
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
       const int gk = k0 + (g & 7), gn = n0 + 2 * (g >> 3);
-      const unsigned off = AM == 0 ? unsigned(gk) * unsigned(N) + unsigned(gn)
-                                   : b_kbase + unsigned(b_roff) + unsigned(b_off[h]);
+      const unsigned off = AM == 0 ? unsigned(gk - k0) * unsigned(N) + unsigned(gn - n0)
+                                   : unsigned(b_roff) + unsigned(b_off[h]);
       const bool in = (kBT % NTH == 0 || g < kBT) && gk < K && gn < N;
       rb[h] = load16(rsrc_b, in ? off * unsigned(sizeof(float2)) : kOutside);
     }
@@ -1008,10 +1037,36 @@ hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B
 enum BigLayout { kLayoutNone = 0, kLayout4x2, kLayout2x4, kLayout8x1 };
 
 // The large-tile kernel addresses each operand with 32-bit byte offsets
-// into a buffer resource.
+// into a buffer resource based at the block's first row of A (first column
+// of B) and at the stage's first k: one stage of a block reads at most 512
+// rows of A over 8 k, and 8 rows of B over at most 256 complex columns.
+constexpr uint64_t kSpanLimit = (uint64_t(1) << 31) - 64;
 bool big_operands_fit(uint64_t m, uint64_t n, uint64_t k) {
-  const uint64_t lim = (uint64_t(1) << 31) - 64;
-  return m * k * sizeof(float2) <= lim && k * n * sizeof(float2) <= lim;
+  return 512 * k * sizeof(float2) <= kSpanLimit && 8 * n * sizeof(float2) <= kSpanLimit &&
+         m <= 0x7fffffffULL && n <= 0x7fffffffULL && k <= 0x7fffffffULL;
+}
+
+static uint64_t map_offset_host(const qfx_index_map& d, uint64_t idx) {
+  uint64_t off = 0;
+  for (int j = 0; j < d.ndigits; ++j) {
+    off += (idx & ((uint64_t(1) << d.bits[j]) - 1)) << d.shift[j];
+    idx >>= d.bits[j];
+  }
+  return off;
+}
+
+// Digit-mapped operands: what one stage of one block reads (every tile
+// shape: at most 512 rows of A and 256 complex columns of B, 8 k) must fit
+// the buffer offsets.
+bool big_mapped_spans_fit(const qfx_index_map& a_rows, const qfx_index_map& a_cols,
+                          const qfx_index_map& b_rows, const qfx_index_map& b_cols, uint64_t m,
+                          uint64_t n, uint64_t k) {
+  const uint64_t a_span = map_offset_host(a_rows, std::min<uint64_t>(512, m) - 1) +
+                          map_offset_host(a_cols, std::min<uint64_t>(kBBK, k) - 1) + 1;
+  const uint64_t b_span = map_offset_host(b_cols, std::min<uint64_t>(256, n) - 1) +
+                          map_offset_host(b_rows, std::min<uint64_t>(kBBK, k) - 1) + 1;
+  return a_span * sizeof(float2) <= kSpanLimit && b_span * sizeof(float2) <= kSpanLimit &&
+         m <= 0x7fffffffULL && n <= 0x7fffffffULL && k <= 0x7fffffffULL;
 }
 
 BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
@@ -2128,7 +2183,7 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
   const bool split = precision == QFX_PREC_SPLIT_FP16 || precision == QFX_PREC_SPLIT_BF16 ||
                      precision == QFX_PREC_SPLIT_BF16X3;
   const BigLayout layout =
-      (split && big_operands_fit(m, n, k) && k >= 8 &&
+      (split && big_mapped_spans_fit(*a_rows, *a_cols, *b_rows, *b_cols, m, n, k) && k >= 8 &&
        reinterpret_cast<uintptr_t>(A) % 16 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0)
           ? choose_big_layout(precision, m, n)
           : kLayoutNone;

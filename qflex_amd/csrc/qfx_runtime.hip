@@ -528,6 +528,39 @@ extern "C" int qfx_accumulate_c128(qfx_context* ctx, const qfx_c64* src, uint64_
   return QFX_OK;
 }
 
+// amps[i] += src[b][i] for b = 0, 1, ..., nbatch - 1, one FP64 addition at a
+// time in that order (the order of the per-slice accumulations it replaces).
+__global__ void qfx_accumulate_batched_kernel(const float2* __restrict__ src, uint64_t n,
+                                              uint64_t nbatch, double2* __restrict__ amps) {
+  uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double2 a = amps[i];
+  for (uint64_t b = 0; b < nbatch; ++b) {
+    const float2 v = src[b * n + i];
+    a.x += double(v.x);
+    a.y += double(v.y);
+  }
+  amps[i] = a;
+}
+
+extern "C" int qfx_accumulate_batched_c128(qfx_context* ctx, const qfx_c64* src, uint64_t n,
+                                           uint64_t nbatch, qfx_c128* amps, uint64_t offset) {
+  QFX_CHECK_CTX(ctx);
+  if (!n || !nbatch) return QFX_OK;
+  if (!src || !amps)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_accumulate_batched_c128: null pointer");
+  if (n > UINT64_MAX / nbatch || (n + 255) / 256 > 0x7fffffffULL)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_accumulate_batched_c128: size too large");
+  QFX_EXTENT(ctx, src, 8.0 * double(n) * double(nbatch), "qfx_accumulate_batched_c128: src");
+  QFX_EXTENT(ctx, amps + offset, 16.0 * double(n), "qfx_accumulate_batched_c128: amps");
+  QfxLaunchScope scope(ctx, QFX_K_ACCUMULATE, 8.0 * n * nbatch + 32.0 * n, 2.0 * n * nbatch);
+  const unsigned g = unsigned((n + 255) / 256);
+  qfx_accumulate_batched_kernel<<<g, 256, 0, ctx->stream>>>(
+      reinterpret_cast<const float2*>(src), n, nbatch, reinterpret_cast<double2*>(amps) + offset);
+  QFX_HIP(hipGetLastError());
+  return QFX_OK;
+}
+
 // ---------------------------------------------------------------- graphs
 
 extern "C" int qfx_graph_begin(qfx_context* ctx, size_t workspace_bytes) {
