@@ -164,6 +164,38 @@ def cpu_baseline(circ, grid):
             "extrapolated": True}
 
 
+def run_queue(qflex, states, precision):
+    """The uneven queue of qflex_amd.jobqueue: each job (one circuit, `states`
+    base bitstrings, 64 amplitudes each) run once untimed (tile autotuning),
+    then timed; amplitudes/s per job and for the whole queue."""
+    from qflex_amd import jobqueue
+    jobs = jobqueue.queue(states=states)
+    out = {"metric": "amplitudes/s, uneven queue (6x6 d24, 7x7 d32, 8x7 d24; 64-amplitude open "
+                     "region per base bitstring)", "precision": precision, "jobs": []}
+    total_amps, total_s = 0, 0.0
+    for job in jobs:
+        opts = dict(job["options"], precision=precision, autotune=AUTOTUNE)
+        warm = dict(opts, final_states=opts["final_states"][:1])
+        qflex.evaluate(warm)
+        qflex.synchronize()
+        t0 = time.perf_counter()
+        r = qflex.evaluate(opts)
+        qflex.synchronize()
+        dt = time.perf_counter() - t0
+        st = qflex.last_stats()
+        total_amps += len(r)
+        total_s += dt
+        out["jobs"].append({"job": job["name"], "base_bitstrings": job["states"],
+                            "amplitudes": len(r), "seconds": dt, "amplitudes_per_s": len(r) / dt,
+                            "flops": st["flops"], "slices": st["slices"],
+                            "slice_batch": opts["slice_batch"]})
+        log(f"[bench] queue {job['name']}: {len(r)} amplitudes in {dt:.2f} s")
+    out["value"] = total_amps / total_s
+    out["unit"] = "amplitudes/s"
+    out["seconds"] = total_s
+    return out
+
+
 def microbench(args):
     """Times qfx_cgemm_c64 on fixed shapes (random complex N(0,1) operands)."""
     import numpy as np
@@ -272,6 +304,7 @@ def summarize_pmc(fetch_dir, write_dir, copy_mib):
 
 
 def main():
+    global AUTOTUNE, SLICE_BATCH
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
@@ -288,6 +321,10 @@ def main():
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
                          "with HIP events through the C-ABI, each --steps times per variant); "
                          "prints one JSON line and skips the workload")
+    ap.add_argument("--queue", type=int, default=0, metavar="STATES",
+                    help="queue mode: the uneven queue of qflex_amd.jobqueue (6x6, 7x7, 8x7 "
+                         "grids, a six-qubit open region, STATES base bitstrings each) in "
+                         "split_fp16; prints one JSON line and skips the workload")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                     help="summarize two rocprofv3 --pmc output folders (FETCH_SIZE, WRITE_SIZE) "
                          "of a run with --calibrate-copy-mib into HBM bytes per MFMA cGEMM "
@@ -296,7 +333,6 @@ def main():
                     help="first run one device copy of this many MiB (qfx_copy_kernel), the "
                          "known-size reference for HBM byte counters")
     args = ap.parse_args()
-    global AUTOTUNE, SLICE_BATCH
     AUTOTUNE = not args.no_autotune
     SLICE_BATCH = [int(x) for x in args.slice_batch.split(",")]
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -322,6 +358,11 @@ def main():
 
     if args.microbench:
         print(json.dumps(microbench(args)))
+        return
+
+    if args.queue:
+        print(json.dumps(run_queue(qflex, args.queue, args.variants.split(",")[0]
+                                   if args.variants != ap.get_default("variants") else "split_fp16")))
         return
 
     circ, order, grid = workload(args.seed)
