@@ -268,7 +268,8 @@ std::unordered_map<std::string, std::size_t> bond_cut_batches(
 }
 
 SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
-                              const std::vector<std::vector<Tensor>>& tensor_grid) {
+                              const std::vector<std::vector<Tensor>>& tensor_grid,
+                              const std::vector<std::string>& projected) {
   SymbolicPlan plan;
   std::unordered_map<std::string, std::size_t> index_dimension;
   std::vector<std::vector<std::vector<std::string>>> grid_indices(tensor_grid.size());
@@ -276,7 +277,7 @@ SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
     grid_indices[i].resize(tensor_grid[i].size());
     for (std::size_t j = 0; j < tensor_grid[i].size(); ++j) {
       const Tensor& t = tensor_grid[i][j];
-      grid_indices[i][j] = t.get_indices();
+      grid_indices[i][j] = _vector_subtraction(t.get_indices(), projected);
       for (const auto& index : t.get_indices())
         index_dimension[index] = t.get_index_to_dimension().at(index);
       if (t.data()) plan.max_grid = std::max(plan.max_grid, t.size());
@@ -388,14 +389,15 @@ SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
 
 ContractionData ContractionData::Initialize(const std::list<ContractionOperation>& ordering,
                                             std::vector<std::vector<Tensor>>* tensor_grid,
-                                            std::vector<std::complex<double>>* amplitudes) {
+                                            std::vector<std::complex<double>>* amplitudes,
+                                            const std::vector<std::string>& projected) {
   if (tensor_grid == nullptr) throw ERROR_MSG("Tensor grid must be non-null.");
   if (amplitudes == nullptr) throw ERROR_MSG("Amplitude return vector must be non-null.");
   ContractionData data;
   data.tensor_grid_ = tensor_grid;
   data.amplitudes_ = amplitudes;
 
-  SymbolicPlan plan = plan_contraction(ordering, *tensor_grid);
+  SymbolicPlan plan = plan_contraction(ordering, *tensor_grid, projected);
   data.patch_max_size_ = plan.patch_max_size;
   const std::size_t max_grid = plan.max_grid;
   const auto& side_buffers = plan.side_buffers;
@@ -1006,39 +1008,57 @@ void ContractionData::run(std::list<ContractionOperation>& ordering, std::size_t
     grid_views_[it->first[0]][it->first[1]] = it->second;
 }
 
+// Start of a top-level contraction: grid views of the grid tensors, fresh
+// state and counters, the device drained (the network build and flatten
+// before it are not counted), the FP64 amplitudes zeroed.
+void ContractionData::begin_contraction() {
+  grid_views_.assign(tensor_grid_->size(), {});
+  for (std::size_t i = 0; i < tensor_grid_->size(); ++i) {
+    grid_views_[i].resize((*tensor_grid_)[i].size());
+    for (std::size_t j = 0; j < (*tensor_grid_)[i].size(); ++j) {
+      const Tensor& t = (*tensor_grid_)[i][j];
+      grid_views_[i][j].ptr = t.data();
+      grid_views_[i][j].indices = t.get_indices();
+      grid_views_[i][j].dims = t.get_dimensions();
+    }
+  }
+  patches_.clear();
+  batch_.clear();
+  grid_amax_.clear();
+  grid_amax_store_.clear();
+  grid_ranges_.clear();
+  for (const auto& row : *tensor_grid_)
+    for (const auto& t : row) grid_ranges_.push_back({t.data(), t.data() + t.capacity()});
+  ContractionStats fresh;
+  fresh.planned_bytes = stats_.planned_bytes;
+  stats_ = fresh;
+  qfx_check(qfx_synchronize(device()), "qfx_synchronize");
+  start_ = std::chrono::steady_clock::now();
+  Tensor& amps = get_scratch(kAmplitudeSpace);
+  qfx_check(qfx_zero(device(), amps.data(), amps.capacity() * sizeof(s_type)), "qfx_zero");
+}
+
+// End of a top-level contraction: the FP64 amplitudes downloaded once and
+// added to the caller's vector; wall time and counters recorded.
+void ContractionData::end_contraction() {
+  std::vector<std::complex<double>> device_amps(amplitudes_->size());
+  if (!device_amps.empty())
+    qfx_check(qfx_copy_d2h(device(), device_amps.data(), get_scratch(kAmplitudeSpace).data(),
+                           device_amps.size() * sizeof(std::complex<double>)),
+              "qfx_copy_d2h");
+  for (std::size_t i = 0; i < device_amps.size(); ++i) (*amplitudes_)[i] += device_amps[i];
+  qfx_check(qfx_synchronize(device()), "qfx_synchronize");
+  stats_.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - start_).count();
+  g_last_stats = stats_;
+}
+
 void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
                                    std::size_t output_index,
                                    std::unordered_map<std::string, bool> active_patches) {
   const bool top = depth_ == 0;
   if (top) {
-    grid_views_.assign(tensor_grid_->size(), {});
-    for (std::size_t i = 0; i < tensor_grid_->size(); ++i) {
-      grid_views_[i].resize((*tensor_grid_)[i].size());
-      for (std::size_t j = 0; j < (*tensor_grid_)[i].size(); ++j) {
-        const Tensor& t = (*tensor_grid_)[i][j];
-        grid_views_[i][j].ptr = t.data();
-        grid_views_[i][j].indices = t.get_indices();
-        grid_views_[i][j].dims = t.get_dimensions();
-      }
-    }
-    patches_.clear();
     cut_batches_ = bond_cut_batches(ordering, *tensor_grid_);
-    batch_.clear();
-    grid_amax_.clear();
-    grid_amax_store_.clear();
-    grid_ranges_.clear();
-    for (const auto& row : *tensor_grid_)
-      for (const auto& t : row)
-        grid_ranges_.push_back({t.data(), t.data() + t.capacity()});
-    ContractionStats fresh;
-    fresh.planned_bytes = stats_.planned_bytes;
-    stats_ = fresh;
-    // wall time of the contraction itself: earlier device work (network
-    // build, flatten) is finished first
-    qfx_check(qfx_synchronize(device()), "qfx_synchronize");
-    start_ = std::chrono::steady_clock::now();
-    Tensor& amps = get_scratch(kAmplitudeSpace);
-    qfx_check(qfx_zero(device(), amps.data(), amps.capacity() * sizeof(s_type)), "qfx_zero");
+    begin_contraction();
   }
   ++depth_;
   try {
@@ -1048,18 +1068,43 @@ void ContractionData::ContractGrid(std::list<ContractionOperation> ordering,
     throw;
   }
   --depth_;
-  if (top) {
-    std::vector<std::complex<double>> device_amps(amplitudes_->size());
-    if (!device_amps.empty())
-      qfx_check(qfx_copy_d2h(device(), device_amps.data(), get_scratch(kAmplitudeSpace).data(),
-                             device_amps.size() * sizeof(std::complex<double>)),
-                "qfx_copy_d2h");
-    for (std::size_t i = 0; i < device_amps.size(); ++i) (*amplitudes_)[i] += device_amps[i];
-    qfx_check(qfx_synchronize(device()), "qfx_synchronize");
-    stats_.seconds =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - start_).count();
-    g_last_stats = stats_;
+  if (top) end_contraction();
+}
+
+void ContractionData::ContractGridStates(const std::list<ContractionOperation>& ordering,
+                                         const std::unordered_map<std::string, bool>& active_patches,
+                                         const std::vector<std::vector<SiteProjection>>& states) {
+  if (depth_ != 0) throw ERROR_MSG("ContractGridStates cannot be nested.");
+  cut_batches_ = bond_cut_batches(ordering, *tensor_grid_);
+  begin_contraction();
+  const auto base = grid_views_;
+  for (std::size_t s = 0; s < states.size(); ++s) {
+    grid_views_ = base;
+    for (const auto& p : states[s]) {
+      if (p.i >= grid_views_.size() || p.j >= grid_views_[p.i].size())
+        throw ERROR_MSG("Projection of a tensor outside the grid: (", p.i, ",", p.j, ").");
+      try {
+        grid_views_[p.i][p.j] = project_view(base[p.i][p.j], p.index, p.value, nullptr, nullptr);
+      } catch (const std::string& err_msg) {
+        throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+      }
+    }
+    // the operand maxima cached for grid views belong to this state's views
+    grid_amax_.clear();
+    grid_amax_store_.clear();
+    patches_.clear();
+    std::list<ContractionOperation> rest(ordering);
+    auto active = active_patches;
+    ++depth_;
+    try {
+      run(rest, s, active, {});
+    } catch (...) {
+      --depth_;
+      throw;
+    }
+    --depth_;
   }
+  end_contraction();
 }
 
 void ContractGrid(const std::list<ContractionOperation>& ordering,

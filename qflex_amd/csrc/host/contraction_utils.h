@@ -118,8 +118,12 @@ struct SymbolicPlan {
   std::size_t max_grid = 0;
   std::vector<ContractionStep> steps;
 };
+// `projected`: indices of the grid tensors that are fixed to one value
+// before the contraction (the open output indices of a network flattened
+// once for several final states); the pass leaves them out.
 SymbolicPlan plan_contraction(const std::list<ContractionOperation>& ordering,
-                              const std::vector<std::vector<Tensor>>& tensor_grid);
+                              const std::vector<std::vector<Tensor>>& tensor_grid,
+                              const std::vector<std::string>& projected = {});
 
 // Batch size of every bond cut of `ordering` under global::slice_batch, by
 // cut index name (at most the cut's value count; 1 = one value at a time).
@@ -131,7 +135,8 @@ class ContractionData {
  public:
   static ContractionData Initialize(const std::list<ContractionOperation>& ordering,
                                     std::vector<std::vector<Tensor>>* tensor_grid,
-                                    std::vector<std::complex<double>>* amplitudes);
+                                    std::vector<std::complex<double>>* amplitudes,
+                                    const std::vector<std::string>& projected = {});
 
   static constexpr char kGeneralSpace[] = "_general_internal_";
   static constexpr char kGeneralSpaceB[] = "_general_internal_b_";
@@ -140,6 +145,20 @@ class ContractionData {
 
   void ContractGrid(std::list<ContractionOperation> ordering, std::size_t output_index,
                     std::unordered_map<std::string, bool> active_patches);
+  // One index of one grid tensor fixed to a value (a zero-copy view: the
+  // index must be the tensor's first).
+  struct SiteProjection {
+    std::size_t i, j;
+    std::string index;
+    std::size_t value;
+  };
+  // The whole ordering once per entry of `states` (each a list of grid
+  // tensor projections, e.g. the output bits of one final state), state s
+  // writing its amplitudes at output index s; the amplitudes are zeroed
+  // once and downloaded once.
+  void ContractGridStates(const std::list<ContractionOperation>& ordering,
+                          const std::unordered_map<std::string, bool>& active_patches,
+                          const std::vector<std::vector<SiteProjection>>& states);
 
   Tensor& get_scratch(std::string id) { return scratch_[scratch_map_.at(id)]; }
 
@@ -196,6 +215,8 @@ class ContractionData {
   std::unordered_map<std::string, View> patches_;
   int depth_ = 0;
   ContractionStats stats_;
+  void begin_contraction();
+  void end_contraction();
   // batch size of each bond cut (by index name), and the cut indices whose
   // current values are a batch (kept open, never summed by a GEMM)
   std::unordered_map<std::string, std::size_t> cut_batches_;

@@ -29,6 +29,17 @@ std::string get_output_states(const QflexInput* input,
 // contracts it and returns (bitstring, amplitude) pairs.
 std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuit(QflexInput* input);
 
+// Amplitudes of several final states of one circuit (no counterpart in the
+// reference, whose simulate() takes one final state,
+// src/pybind_main.cpp:112-113).  The network is built and flattened once,
+// with an open output index on every qubit whose bit differs between the
+// states (besides the ordering's terminal cuts); each state then contracts
+// zero-copy projections of those sites' tensors onto its bits.  Returns, per
+// state in order, the (bitstring, amplitude) pairs EvaluateCircuit would
+// return for it.
+std::vector<std::pair<std::string, std::complex<double>>> EvaluateCircuitStates(
+    QflexInput* input, const std::vector<std::string>& final_states);
+
 // The same network build, then only the symbolic pass of the contraction
 // (pairwise-contraction shapes and multiplicities, patch sizes): the
 // features of the learned cost model.

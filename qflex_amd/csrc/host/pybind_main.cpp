@@ -10,7 +10,9 @@
 // on first use and keep the fastest, for this and later calls) and
 // "slice_batch" (int, or one int per bond cut in ordering order: the number
 // of cut values contracted together, the cut index kept as a batch index;
-// default 1, the reference's one slice at a time).
+// default 1, the reference's one slice at a time).  "final_states" may list
+// several bitstrings (the reference accepts one): the network is then built
+// and flattened once and contracted once per state.
 //
 // The remaining functions expose the host's Tensor / contraction API to the
 // tests; they raise RuntimeError with the reference's message texts.
@@ -100,7 +102,8 @@ int parse_precision(const std::string& p) {
 }
 
 // Fills a QflexInput (and the process-wide settings) from a simulate() dict.
-void load_input(const py::dict& options, qflex::QflexInput& input) {
+void load_input(const py::dict& options, qflex::QflexInput& input,
+                std::vector<std::string>* final_states = nullptr) {
   if (options.contains("memory_limit")) {
     const auto& ml = options["memory_limit"];
     if (py::isinstance<py::int_>(ml)) {
@@ -141,16 +144,20 @@ void load_input(const py::dict& options, qflex::QflexInput& input) {
   read_input(options, "circuit", input.circuit);
   read_input(options, "ordering", input.ordering);
   const auto initial_states = read_states(options, "initial");
-  const auto final_states = read_states(options, "final");
-  if (initial_states.size() != 1 || final_states.size() != 1)
-    throw ERROR_MSG("Not yet supported");
+  const auto finals = read_states(options, "final");
+  // One initial state, as the reference; several final states are an
+  // extension (EvaluateCircuitStates: one network build for all of them).
+  if (initial_states.size() != 1 || finals.empty()) throw ERROR_MSG("Not yet supported");
   input.initial_state = initial_states[0];
-  input.final_state = final_states[0];
+  input.final_state = finals[0];
+  if (final_states) *final_states = finals;
 }
 
 std::vector<std::pair<std::string, std::complex<double>>> evaluate(const py::dict& options) {
   qflex::QflexInput input;
-  load_input(options, input);
+  std::vector<std::string> finals;
+  load_input(options, input, &finals);
+  if (finals.size() > 1) return qflex::EvaluateCircuitStates(&input, finals);
   return qflex::EvaluateCircuit(&input);
 }
 

@@ -327,3 +327,13 @@ def test_costmodel_features_from_a_plan():
     assert abs(f[0] - np.log2(total)) < 1e-5
     assert f[2] == 20 and f[3] == 5 and f[6] == 4
     assert abs(f[8:16].sum() - 1) < 1e-6 and abs(f[32:40].sum() - 1) < 1e-6
+
+
+def test_several_final_states_of_unequal_length_are_refused():
+    """simulate's final_states list (one network build for all states): a
+    state of the wrong length is refused before any device work."""
+    opts = {"circuit": ["2", "0 h 0", "0 h 1", "1 cz 0 1"], "grid": ["11"],
+            "ordering": ["expand a 0", "expand a 1"], "initial_state": "00",
+            "final_states": ["01", "011"]}
+    with pytest.raises(RuntimeError, match="Final state '011' has 3 qubits instead of 2"):
+        qflex.evaluate(opts)
