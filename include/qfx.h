@@ -135,6 +135,22 @@ int qfx_reorder_c64(qfx_context* ctx, const qfx_c64* src, qfx_c64* dst, int rank
 int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n, uint64_t k,
                   const qfx_c64* A, const qfx_c64* B, qfx_c64* C);
 
+/* qfx_cgemm_c64 that allocates no device memory: no counterpart in the
+ * reference.  For callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The kernels' scratch
+ * (split-K partial sums, operand maxima, FP8 codes) lives in a fixed device
+ * array of 16 MiB per GPU (QFX_NOALLOC_SCRATCH_BYTES; each GPU the library
+ * runs on holds it until the process ends) instead of the context's growable
+ * workspace; a product that needs more scratch than that returns
+ * QFX_ERR_UNSUPPORTED before anything is launched for it.  The large tiles
+ * follow the fixed rule (the autotuner times nothing here); results are
+ * bit-identical to qfx_cgemm_c64's.  Every such call on one GPU shares that
+ * GPU's array, which keeps the last call's contents: calls that overlap on
+ * one GPU (contexts on different streams) race. */
+#define QFX_NOALLOC_SCRATCH_BYTES (16u << 20)
+int qfx_cgemm_noalloc_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n, uint64_t k,
+                          const qfx_c64* A, const qfx_c64* B, qfx_c64* C);
+
 /* Independent products C_i = A_i . B_i (row-major, as qfx_cgemm_c64) of any
  * shapes.  In the split precisions the products with k > 16, even k and n
  * and 16-byte aligned operands run as ONE persistent launch of the

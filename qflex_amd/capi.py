@@ -71,6 +71,7 @@ SIGNATURES = {
     "qfx_reorder_c64": (_c.c_int, [_vp, _vp, _vp, _c.c_int, _c.POINTER(_u64),
                                    _c.POINTER(_c.c_int)]),
     "qfx_cgemm_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _u64, _vp, _vp, _vp]),
+    "qfx_cgemm_noalloc_c64": (_c.c_int, [_vp, _c.c_int, _u64, _u64, _u64, _vp, _vp, _vp]),
     "qfx_cgemm_mapped_c64": (_c.c_int, [_vp, _c.c_int, _vp, _c.POINTER(IndexMap),
                                         _c.POINTER(IndexMap), _vp, _c.POINTER(IndexMap),
                                         _c.POINTER(IndexMap), _vp]),
@@ -218,6 +219,10 @@ class Context:
 
     def cgemm(self, precision, m, n, k, A, B, C):
         check(load().qfx_cgemm_c64(self.handle, precision, m, n, k, A.ptr, B.ptr, C.ptr))
+
+    def cgemm_noalloc(self, precision, m, n, k, A, B, C):
+        """qfx_cgemm_noalloc_c64: the same product, scratch from the fixed array."""
+        check(load().qfx_cgemm_noalloc_c64(self.handle, precision, m, n, k, A.ptr, B.ptr, C.ptr))
 
     def cgemm_mapped(self, precision, A, a_rows, a_cols, B, b_rows, b_cols, C):
         """Returns False (C untouched) when no kernel reads this form."""

@@ -2029,6 +2029,30 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   return QFX_OK;
 }
 
+// The same product with no device allocation: every kernel's scratch
+// (split-K partial sums, operand maxima, FP8 codes) comes from the fixed
+// scratch array of the context's device, the large tiles follow the fixed
+// rule (no autotuner timing), and a product that needs more scratch than the
+// array holds is refused before anything is launched for it.
+extern "C" int qfx_cgemm_noalloc_c64(qfx_context* ctx, int precision, uint64_t m, uint64_t n,
+                                     uint64_t k, const qfx_c64* A, const qfx_c64* B,
+                                     qfx_c64* C) {
+  QFX_CHECK_CTX(ctx);
+  if (!ctx->fixed_scratch)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_noalloc_c64: no fixed scratch on this device");
+  struct Restore {
+    qfx_context* c;
+    bool autotune;
+    ~Restore() {
+      c->use_fixed_scratch = false;
+      c->autotune = autotune;
+    }
+  } restore{ctx, ctx->autotune};
+  ctx->use_fixed_scratch = true;
+  ctx->autotune = false;
+  return qfx_cgemm_c64(ctx, precision, m, n, k, A, B, C);
+}
+
 // ------------------------------------------------------------ mapped operands
 
 static bool map_size(const qfx_index_map* d, uint64_t* size) {

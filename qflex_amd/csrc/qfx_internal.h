@@ -68,7 +68,17 @@ struct qfx_context {
   bool autotune = false;
   std::map<std::array<uint64_t, 5>, int> tuned;
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
+  // This device's copy of the library's fixed scratch array (the device
+  // address hipGetSymbolAddress returned at context creation), and whether
+  // the call in progress must take its workspace from it instead of the
+  // growable workspace above (qfx_cgemm_noalloc_c64).
+  void* fixed_scratch = nullptr;
+  bool use_fixed_scratch = false;
 };
+
+// Size of the fixed scratch array (one copy per GPU, held until the process
+// ends) that the entry point allocating nothing uses as its workspace.
+constexpr size_t kFixedScratchBytes = QFX_NOALLOC_SCRATCH_BYTES;
 
 // Error plumbing: every entry point returns a status and records a message.
 int qfx_set_error(int status, const std::string& msg);

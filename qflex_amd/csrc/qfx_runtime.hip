@@ -21,6 +21,31 @@ extern "C" const char* qfx_last_error(void) { return g_last_error.c_str(); }
 
 // ---------------------------------------------------------------- context
 
+// Workspace of the entry point that allocates nothing
+// (qfx_cgemm_noalloc_c64): a fixed array of which every GPU holds its own
+// copy from module load until the process ends.  Host code never uses the
+// array's name as an address: it uses what hipGetSymbolAddress returns for
+// the device (fixed_scratch_address).
+static __device__ __align__(256) unsigned char scratch[kFixedScratchBytes];
+
+// The device address of this device's `scratch`, looked up once per device
+// with that device current (the caller has set it).
+static int fixed_scratch_address(int device, void** out) {
+  static std::mutex mu;
+  static std::map<int, void*> per_device;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = per_device.find(device);
+  if (it == per_device.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return qfx_hip_error(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return qfx_set_error(QFX_ERR_HIP, "hipGetSymbolAddress(scratch) returned null");
+    it = per_device.emplace(device, p).first;
+  }
+  *out = it->second;
+  return QFX_OK;
+}
+
 extern "C" int qfx_context_create(int device, size_t arena_chunk_bytes, qfx_context** out) {
   if (!out) return qfx_set_error(QFX_ERR_INVALID, "null output pointer");
   int n = 0;
@@ -29,8 +54,12 @@ extern "C" int qfx_context_create(int device, size_t arena_chunk_bytes, qfx_cont
     return qfx_set_error(QFX_ERR_INVALID, "device " + std::to_string(device) +
                                               " out of range (devices: " + std::to_string(n) + ")");
   QFX_HIP(hipSetDevice(device));
+  void* fixed = nullptr;
+  const int fs = fixed_scratch_address(device, &fixed);
+  if (fs != QFX_OK) return fs;
   auto* ctx = new qfx_context();
   ctx->device = device;
+  ctx->fixed_scratch = fixed;
   if (arena_chunk_bytes) ctx->chunk_bytes = arena_chunk_bytes;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -167,6 +196,16 @@ int qfx_check_extent(qfx_context* ctx, const void* p, double bytes, const char* 
 }
 
 int qfx_workspace(qfx_context* ctx, size_t bytes, void** out) {
+  if (ctx->use_fixed_scratch) {
+    // the no-allocation entry point: the fixed scratch array or nothing
+    if (bytes > kFixedScratchBytes)
+      return qfx_set_error(QFX_ERR_UNSUPPORTED,
+                           "qfx_cgemm_noalloc_c64: this product needs " + std::to_string(bytes) +
+                               " bytes of device scratch, more than the fixed " +
+                               std::to_string(kFixedScratchBytes) + " bytes");
+    *out = ctx->fixed_scratch;
+    return QFX_OK;
+  }
   if (ctx->work_bytes < bytes && ctx->capturing)
     return qfx_set_error(QFX_ERR_INVALID,
                          "device workspace of " + std::to_string(bytes) +
