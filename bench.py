@@ -55,9 +55,11 @@ def workload(seed):
 
 
 AUTOTUNE = True  # tile autotuner on (the warmup step does the timing of new products)
-# Values of each bond cut contracted together (simulate's "slice_batch"): the
-# inner cut's 16 values as one batch, the outer cut one value at a time.
-SLICE_BATCH = [1, 16]
+# Values of each bond cut contracted together (simulate's "slice_batch"):
+# one value at a time.  (Batching the inner cut's 16 values measured no
+# faster: its stacked operands of 2-8 GiB no longer stay in the 256 MB
+# last-level cache between consecutive products, as the per-slice ones do.)
+SLICE_BATCH = [1]
 
 
 def options(circ, order, grid, precision):

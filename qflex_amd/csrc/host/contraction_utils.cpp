@@ -774,7 +774,7 @@ ContractionData::View ContractionData::contract_pair(const View& A, const View& 
 ContractionData::View ContractionData::project_view(const View& v, const std::string& index,
                                                     std::size_t value,
                                                     const std::vector<std::size_t>* gather_values,
-                                                    s_type* buffer) {
+                                                    s_type* buffer, bool copy) {
   auto it = std::find(v.indices.begin(), v.indices.end(), index);
   if (it == v.indices.end())
     throw ERROR_MSG("Index: '", index, "' has to be equal to indices[0]: '",
@@ -803,7 +803,7 @@ ContractionData::View ContractionData::project_view(const View& v, const std::st
   }
   bool consecutive = true;
   for (std::size_t j = 1; j < picks.size(); ++j) consecutive = consecutive && picks[j] == picks[0] + j;
-  if (pre == 1 && consecutive && !picks.empty()) {
+  if (pre == 1 && consecutive && !picks.empty() && !copy) {
     out.ptr = v.ptr + picks[0] * post;  // contiguous slice: zero copy
     return out;
   }
@@ -866,6 +866,95 @@ void ContractionData::emit_output(const View& output, std::size_t output_index,
   stats_.slices += nb;
 }
 
+// A bond cut's passes.  A batched pass keeps the cut index, restricted to
+// the pass's values, as a batch index.  With global::record_slices (and no
+// further bond cut in the rest of the ordering, no profiling, at least three
+// passes) the projections go into the cut's fixed buffers, the first pass
+// runs eagerly (it grows the workspace and tunes the tiles), the second is
+// recorded as one graph, and every later pass is the projection copies
+// plus one launch of that graph; the host's buffer assignment is reset
+// before every pass so that all passes use the same buffers.
+void ContractionData::cut_loop(const std::string& index, const std::vector<std::size_t>& pa,
+                               const std::vector<std::size_t>& pb, const View& saved_a,
+                               const View& saved_b, s_type* buf_a, s_type* buf_b,
+                               const std::vector<std::vector<std::size_t>>& groups, bool batched,
+                               const std::list<ContractionOperation>& ordering,
+                               std::size_t output_index,
+                               const std::unordered_map<std::string, bool>& active_patches,
+                               const std::vector<OpenOutput>& open_outputs) {
+  bool innermost = true;
+  for (const auto& op : ordering)
+    if (op.op_type == ContractionOperation::CUT && op.cut.tensors.size() > 1) innermost = false;
+  const bool record = global::record_slices && !global::profiling && innermost && !recording_ &&
+                      groups.size() >= 3;
+  const auto map_before = scratch_map_;
+  qfx_graph* graph = nullptr;
+  ContractionStats per_pass{};
+  if (batched) batch_.push_back(index);
+  auto cleanup = [&]() {
+    if (batched) batch_.pop_back();
+    if (graph) qfx_graph_destroy(graph);
+    graph = nullptr;
+    recording_ = false;
+  };
+  try {
+    for (std::size_t g = 0; g < groups.size(); ++g) {
+      const auto& picks = groups[g];
+      try {
+        grid_views_[pa[0]][pa[1]] =
+            project_view(saved_a, index, picks[0], batched ? &picks : nullptr, buf_a, record);
+        grid_views_[pb[0]][pb[1]] =
+            project_view(saved_b, index, picks[0], batched ? &picks : nullptr, buf_b, record);
+      } catch (const std::string& err_msg) {
+        throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
+      }
+      if (record) scratch_map_ = map_before;
+      // the last pass may hold fewer values than the recorded one: eager
+      const bool same_shape = picks.size() == groups[0].size();
+      if (record && g >= 2 && graph && same_shape) {
+        qfx_check(qfx_graph_launch(device(), graph), "qfx_graph_launch");
+        stats_.multiplies += per_pass.multiplies;
+        stats_.flops += per_pass.flops;
+        stats_.reorders += per_pass.reorders;
+        stats_.reorder_elements += per_pass.reorder_elements;
+        stats_.mapped_multiplies += per_pass.mapped_multiplies;
+        stats_.slices += per_pass.slices;
+        stats_.replayed_passes++;
+        continue;
+      }
+      std::list<ContractionOperation> rest(ordering);
+      auto active = active_patches;
+      if (record && g == 1 && same_shape) {
+        const ContractionStats before = stats_;
+        qfx_check(qfx_graph_begin(device(), 0), "qfx_graph_begin");
+        recording_ = true;
+        try {
+          run(rest, output_index, active, open_outputs);
+        } catch (...) {
+          qfx_graph* dropped = nullptr;
+          if (qfx_graph_end(device(), &dropped) == QFX_OK && dropped) qfx_graph_destroy(dropped);
+          throw;
+        }
+        recording_ = false;
+        qfx_check(qfx_graph_end(device(), &graph), "qfx_graph_end");
+        qfx_check(qfx_graph_launch(device(), graph), "qfx_graph_launch");
+        per_pass.multiplies = stats_.multiplies - before.multiplies;
+        per_pass.flops = stats_.flops - before.flops;
+        per_pass.reorders = stats_.reorders - before.reorders;
+        per_pass.reorder_elements = stats_.reorder_elements - before.reorder_elements;
+        per_pass.mapped_multiplies = stats_.mapped_multiplies - before.mapped_multiplies;
+        per_pass.slices = stats_.slices - before.slices;
+        continue;
+      }
+      run(rest, output_index, active, open_outputs);
+    }
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
 void ContractionData::run(std::list<ContractionOperation>& ordering, std::size_t output_index,
                           std::unordered_map<std::string, bool>& active_patches,
                           std::vector<OpenOutput> open_outputs) {
@@ -918,43 +1007,13 @@ void ContractionData::run(std::list<ContractionOperation>& ordering, std::size_t
           s_type* buf_b = get_scratch(cut_copy_name(op.cut.tensors, 1)).data();
           auto batch_it = cut_batches_.find(index);
           const std::size_t group = batch_it == cut_batches_.end() ? 1 : batch_it->second;
-          if (group <= 1) {
-            for (std::size_t val : values) {
-              try {
-                grid_views_[pa[0]][pa[1]] = project_view(saved_a, index, val, nullptr, buf_a);
-                grid_views_[pb[0]][pb[1]] = project_view(saved_b, index, val, nullptr, buf_b);
-              } catch (const std::string& err_msg) {
-                throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
-              }
-              std::list<ContractionOperation> rest(ordering);
-              auto active = active_patches;
-              run(rest, output_index, active, open_outputs);
-            }
-          } else {
-            // A batch of values at a time: both tensors keep the cut index,
-            // restricted to the batch's values, as a batch index.
-            batch_.push_back(index);
-            for (std::size_t first = 0; first < values.size(); first += group) {
-              const std::vector<std::size_t> picks(
-                  values.begin() + first, values.begin() + std::min(first + group, values.size()));
-              try {
-                grid_views_[pa[0]][pa[1]] = project_view(saved_a, index, 0, &picks, buf_a);
-                grid_views_[pb[0]][pb[1]] = project_view(saved_b, index, 0, &picks, buf_b);
-              } catch (const std::string& err_msg) {
-                batch_.pop_back();
-                throw ERROR_MSG("Failed to call project(). Error:\n\t[", err_msg, "]");
-              }
-              std::list<ContractionOperation> rest(ordering);
-              auto active = active_patches;
-              try {
-                run(rest, output_index, active, open_outputs);
-              } catch (...) {
-                batch_.pop_back();
-                throw;
-              }
-            }
-            batch_.pop_back();
-          }
+          std::vector<std::vector<std::size_t>> groups;
+          for (std::size_t first = 0; first < values.size(); first += std::max<std::size_t>(group, 1))
+            groups.emplace_back(values.begin() + first,
+                                values.begin() + std::min(first + std::max<std::size_t>(group, 1),
+                                                          values.size()));
+          cut_loop(index, pa, pb, saved_a, saved_b, buf_a, buf_b, groups, group > 1, ordering,
+                   output_index, active_patches, open_outputs);
           grid_views_[pa[0]][pa[1]] = saved_a;
           grid_views_[pb[0]][pb[1]] = saved_b;
         } else {

@@ -91,6 +91,7 @@ struct ContractionStats {
   double reorder_elements = 0;
   std::size_t reorders = 0;
   std::size_t mapped_multiplies = 0;  // contractions read without reorder
+  std::size_t replayed_passes = 0;    // cut passes run as a recorded graph
   std::size_t planned_bytes = 0;
   double seconds = 0;  // wall time of the contraction (device work finished)
 };
@@ -225,6 +226,7 @@ class ContractionData {
   // operands (A, B)
   std::vector<Tensor> batch_amax_;
   bool is_batch(const std::string& index) const;
+  bool recording_ = false;  // a pass of a cut loop is being recorded
   View contract_batched(const View& A, const View& B, s_type* out,
                         const std::vector<std::string>& batch_common);
 
@@ -238,7 +240,18 @@ class ContractionData {
                   const std::vector<std::string>& right, s_type* out, qfx_scale_io* io);
   View contract_pair(const View& A, const View& B, s_type* out);
   View project_view(const View& v, const std::string& index, std::size_t value,
-                    const std::vector<std::size_t>* gather_values, s_type* gather_buffer);
+                    const std::vector<std::size_t>* gather_values, s_type* gather_buffer,
+                    bool copy = false);
+  // One bond cut's loop over its values (or batches of values): `groups`
+  // lists the values of each pass; with global::record_slices the passes
+  // after the first replay one recorded sequence.
+  void cut_loop(const std::string& index, const std::vector<std::size_t>& pa,
+                const std::vector<std::size_t>& pb, const View& saved_a, const View& saved_b,
+                s_type* buf_a, s_type* buf_b, const std::vector<std::vector<std::size_t>>& groups,
+                bool batched, const std::list<ContractionOperation>& ordering,
+                std::size_t output_index,
+                const std::unordered_map<std::string, bool>& active_patches,
+                const std::vector<OpenOutput>& open_outputs);
   void emit_output(const View& output, std::size_t output_index,
                    const std::vector<OpenOutput>& open_outputs);
 };

@@ -31,6 +31,15 @@ inline bool mapped_operands = true;
 // amplitudes of the batch's slices are summed in FP64 as one per slice.
 inline std::vector<std::size_t> slice_batch;
 
+// Record the passes of the innermost bond cut's loop: the first pass runs
+// eagerly, the second is recorded as one graph, later passes replay it
+// (the projections of the cut tensors are copied into fixed buffers).
+inline bool record_slices = false;
+
+// Set while the library's per-launch profiling is on (recorded passes would
+// escape it, so nothing is recorded then).
+inline bool profiling = false;
+
 }  // namespace qflex::global
 
 #endif

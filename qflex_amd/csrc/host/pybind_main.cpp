@@ -12,7 +12,8 @@
 // of cut values contracted together, the cut index kept as a batch index;
 // default 1, the reference's one slice at a time).  "final_states" may list
 // several bitstrings (the reference accepts one): the network is then built
-// and flattened once and contracted once per state.
+// and flattened once and contracted once per state.  "record_slices" (bool):
+// the innermost bond cut's passes after the first replay one recorded graph.
 //
 // The remaining functions expose the host's Tensor / contraction API to the
 // tests; they raise RuntimeError with the reference's message texts.
@@ -123,6 +124,8 @@ void load_input(const py::dict& options, qflex::QflexInput& input,
   if (options.contains("precision"))
     qflex::global::precision = parse_precision(options["precision"].cast<std::string>());
   if (options.contains("verbose")) qflex::global::verbose = options["verbose"].cast<int>();
+  qflex::global::record_slices =
+      options.contains("record_slices") && options["record_slices"].cast<bool>();
   qflex::global::slice_batch.clear();
   if (options.contains("slice_batch")) {
     const auto& sb = options["slice_batch"];
@@ -478,6 +481,7 @@ PYBIND11_MODULE(qflex, m) {
     d["reorders"] = s.reorders;
     d["reorder_elements"] = s.reorder_elements;
     d["mapped_multiplies"] = s.mapped_multiplies;
+    d["replayed_passes"] = s.replayed_passes;
     d["seconds"] = s.seconds;
     d["planned_bytes"] = s.planned_bytes;
     return d;
@@ -492,6 +496,7 @@ PYBIND11_MODULE(qflex, m) {
   m.def("profile_enable", [](bool on) {
     wrap([&] {
       qflex::qfx_check(qfx_profile_enable(qflex::device(), on ? 1 : 0), "qfx_profile_enable");
+      qflex::global::profiling = on;
       return 0;
     });
   });
