@@ -323,6 +323,9 @@ def main():
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
                          "with HIP events through the C-ABI, each --steps times per variant); "
                          "prints one JSON line and skips the workload")
+    ap.add_argument("--no-error-check", action="store_true",
+                    help="skip the full-job error measurement against the complex128 "
+                         "amplitudes (for profiler runs that only need the timed steps)")
     ap.add_argument("--queue", type=int, default=0, metavar="STATES",
                     help="queue mode: the uneven queue of qflex_amd.jobqueue (6x6, 7x7, 8x7 "
                          "grids, a six-qubit open region, STATES base bitstrings each) in "
@@ -373,7 +376,8 @@ def main():
         log(f"[bench] variant {prec}: warmup {args.warmup}, steps {args.steps}")
         dt, n_amps, prof, stats, dt_prof = time_variant(qflex, options(circ, order, grid, prec),
                                                         args.steps, args.warmup)
-        err, err_slices, fid = measure_error(qflex, circ, grid, prec)
+        err, err_slices, fid = ((None, 0, None) if args.no_error_check
+                                else measure_error(qflex, circ, grid, prec))
         g = prof["cgemm"]
         p = prof["permute"]
         variants[prec] = {
