@@ -213,10 +213,17 @@ def options(circ_text, order_text, I, J, precision):
 def time_contraction(qflex, opts, repeats=2):
     """Seconds of the contraction (ContractGrid, device work finished) of one
     EvaluateCircuit, best of `repeats`; the network build and flatten, the
-    same for every ordering of a circuit, are not included."""
+    same for every ordering of a circuit, are not included.  An ordering whose
+    tensors the engine refuses to allocate under the memory limit times as
+    infinity."""
     best = float("inf")
     for _ in range(repeats):
-        qflex.evaluate(opts)
+        try:
+            qflex.evaluate(opts)
+        except RuntimeError as e:
+            if "exceeds memory limit" in str(e):
+                return float("inf")
+            raise
         best = min(best, qflex.last_stats()["seconds"])
     return best
 
@@ -282,8 +289,11 @@ def run(args):
             plan = qflex.plan(opts)
             if not ok(plan):
                 continue
+            t = time_contraction(qflex, opts, repeats=1)
+            if not math.isfinite(t):
+                continue
             X.append(features(plan, I, J, depth))
-            T.append(math.log(time_contraction(qflex, opts, repeats=1)))
+            T.append(math.log(t))
         log(f"[costmodel] train circuit {c + 1}/{args.train_circuits}: {len(T)} samples")
     model = CostModel(ctx, args.seed)
     losses = model.fit(np.array(X), np.array(T), steps=args.steps)
@@ -318,14 +328,14 @@ def run(args):
     t_plain = np.array([r["plain_s"] for r in rows])
     t_model = np.array([r["model_s"] for r in rows])
     t_flops = np.array([r["min_flops_s"] for r in rows])
-    fin = np.isfinite(t_plain)
+    fin = np.isfinite(t_plain) & np.isfinite(t_model) & np.isfinite(t_flops)
     report = {
         "train_samples": len(T), "train_circuits": args.train_circuits,
         "candidates_per_circuit": args.candidates, "heldout_circuits": args.heldout,
         "heldout_candidates": args.heldout_candidates, "precision": args.precision,
         "fit_loss_first_last": [losses[0], losses[-1]],
         "geomean_speedup_vs_plain": float(np.exp(np.mean(np.log(t_plain[fin] / t_model[fin])))),
-        "geomean_speedup_vs_min_flops": float(np.exp(np.mean(np.log(t_flops / t_model)))),
+        "geomean_speedup_vs_min_flops": float(np.exp(np.mean(np.log(t_flops[fin] / t_model[fin])))),
         "total_plain_s": float(t_plain[fin].sum()), "total_model_s": float(t_model[fin].sum()),
         "total_min_flops_s": float(t_flops[fin].sum()),
         "model_not_slower_than_plain": float(np.mean(t_model[fin] <= t_plain[fin] * 1.05)),
@@ -337,6 +347,10 @@ def run(args):
         if sel:
             tp = np.array([r["plain_s"] for r in sel])
             tm = np.array([r["model_s"] for r in sel])
+            keep = np.isfinite(tp) & np.isfinite(tm)
+            tp, tm = tp[keep], tm[keep]
+            if not len(tp):
+                continue
             report["per_shape"][f"{I}x{J}x{depth}"] = {
                 "circuits": len(sel), "mean_plain_ms": 1e3 * float(tp.mean()),
                 "mean_model_ms": 1e3 * float(tm.mean()),
