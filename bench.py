@@ -201,6 +201,14 @@ def run_queue(qflex, states, precision):
 def microbench(args):
     """Times qfx_cgemm_c64 on fixed shapes (random complex N(0,1) operands)."""
     import numpy as np
+    # the library calibration runs first, in child processes, before this
+    # process initializes the GPU
+    library = []
+    if args.library_fp16:
+        for shape in args.microbench.split(";"):
+            if not shape.startswith("P:"):
+                m, n, k = (int(x) for x in shape.split(","))
+                library.append(library_fp16_child(m, n, k, args.warmup, args.steps))
     from qflex_amd import capi
     ctx = capi.Context(0)
     rng = np.random.default_rng(0)
@@ -255,7 +263,45 @@ def microbench(args):
         for buf in (A, B, C):
             buf.free()
     ctx.close()
-    return {"microbench": out}
+    return {"microbench": out + library}
+
+
+def library_fp16_child(m, n, k, warmup, steps):
+    """library_fp16 in a child process: torch brings its own HIP runtime,
+    which does not share a process with the one libqfx.so uses."""
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--library-shape", f"{m},{n},{k}",
+                        "--steps", str(steps), "--warmup", str(warmup)],
+                       capture_output=True, text=True, check=True)
+    sys.stderr.write(r.stderr)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def library_fp16(m, n, k, warmup, steps):
+    """Calibration: the vendor library's FP16 GEMM (torch.matmul, hipBLASLt)
+    on the real product one split-FP16 term of an m x n x k complex product
+    performs, A (m x 2k) times the 2 x 2 expansion of B (2k x 2n), FP32
+    accumulation; the split form runs three such products."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(0)
+    a = torch.randn(m, 2 * k, device="cuda", dtype=torch.float16, generator=g)
+    b = torch.randn(2 * k, 2 * n, device="cuda", dtype=torch.float16, generator=g)
+    for _ in range(warmup):
+        torch.matmul(a, b)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        torch.matmul(a, b)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    real_flops = 2.0 * m * (2 * k) * (2 * n)
+    log(f"[bench] {m}x{n}x{k} library FP16 real product {m}x{2 * n}x{2 * k}: {ms:.3f} ms, "
+        f"{real_flops / ms / 1e9:.1f} TF/s; three of them {3 * ms:.3f} ms")
+    return {"m": m, "n": n, "k": k, "variant": "library_fp16_real_product",
+            "real_shape": [m, 2 * n, 2 * k], "avg_ms": ms, "tflops_real": real_flops / ms / 1e9,
+            "three_products_ms": 3 * ms}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cgemm_traffic_7x7_d32.json")
@@ -319,6 +365,10 @@ def main():
     ap.add_argument("--slice-batch", default=",".join(str(x) for x in SLICE_BATCH),
                     help="values of each bond cut contracted together, one per cut in ordering "
                          "order ('1' = one slice at a time, the reference's loop)")
+    ap.add_argument("--library-fp16", action="store_true",
+                    help="with --microbench: also time the vendor library's FP16 GEMM (torch.matmul) "
+                         "on the real product one split-FP16 term performs, for calibration")
+    ap.add_argument("--library-shape", default="", help=argparse.SUPPRESS)
     ap.add_argument("--microbench", default="",
                     help="kernel mode: 'M,N,K' complex GEMM shapes separated by ';' (timed "
                          "with HIP events through the C-ABI, each --steps times per variant); "
@@ -347,6 +397,11 @@ def main():
     if args.summarize_pmc:
         print(json.dumps(summarize_pmc(args.summarize_pmc[0], args.summarize_pmc[1],
                                        args.calibrate_copy_mib or 1024), indent=1))
+        return
+
+    if args.library_shape:
+        m, n, k = (int(x) for x in args.library_shape.split(","))
+        print(json.dumps(library_fp16(m, n, k, args.warmup, args.steps)))
         return
 
     from qflex_amd import qflex
