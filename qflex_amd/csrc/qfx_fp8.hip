@@ -289,6 +289,10 @@ int qfx_cgemm_fp8(qfx_context* ctx, const float2* A, const float2* B, float2* C,
     return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: shape too small for FP8 tiles");
   const uint64_t Mp = round_up(m, kF8BM), Np = round_up(2 * n, kF8BN), Kp = round_up(2 * k, kQT);
   const uint64_t KT = Kp / kQT;
+  // launch limits of the three grids (qfx_fp8_applies keeps every shape it
+  // accepts inside them; checked again here before anything is launched)
+  if (Mp / kQT > 0x7fffffffULL || KT > 65535 || Np / kQT > 0x7fffffffULL || Np / kF8BN > 65535)
+    return qfx_set_error(QFX_ERR_INVALID, "qfx_cgemm_c64: FP8 grid beyond the launch limits");
   const uint64_t a_bytes = Mp * Kp, b_bytes = Np * Kp;
   const uint64_t sa_n = (Mp / kQT) * KT, sb_n = (Np / kQT) * KT;
   void* ws = nullptr;
