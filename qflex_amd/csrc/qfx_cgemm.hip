@@ -630,7 +630,7 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
 // One output tile (rows m0.., complex columns n0..) of the product; the
 // block's 512 threads all call it.  cmax_slot: where the tile's max |C|
 // goes (or null).
-template <int MODE, int WM, int WN, int J, int AM, int I = 2>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2, bool WIDE = false>
 __device__ __forceinline__ void split_big_tile(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
     int N, int K, AmaxRef amax, const qfx_index_map& a_rows, const qfx_index_map& a_cols,
@@ -663,14 +663,25 @@ __device__ __forceinline__ void split_big_tile(
   // A, AM 2:   kk = f & 7, row = 2*(f >> 3) (complex rows row, row+1)
   // B: kk = g & 7 (consecutive lanes take consecutive k, so the LDS stores
   //    are conflict-free), nq = g >> 3 (complex n = 2nq, 2nq+1)
-  // Each operand is read through a buffer resource based at the block's
-  // first row (A) or column (B) and at the stage's first k, so only one
-  // stage's footprint in the block, not the whole operand, has to fit the
-  // 32-bit offsets.  For a digit map, offset(i0 + d) = offset(i0) +
-  // offset(d) when i0 is a multiple of the power-of-two tile edge (or of
-  // the stage depth) and d is below it.
-  uint64_t a_base, b_base, a_span, b_span;  // in elements
-  if constexpr (AM == 0) {
+  // Each operand is read through a buffer resource with 32-bit offsets.
+  // Normally (WIDE false) the resource covers the whole operand, made once.
+  // WIDE: it is based at the block's first row (A) or column (B) and at the
+  // stage's first k, so only one stage's footprint in the block, not the
+  // whole operand, has to fit the offsets (operands above 2 GiB; measured
+  // ~6% slower, so used only where the whole operand does not fit).  For a
+  // digit map, offset(i0 + d) = offset(i0) + offset(d) when i0 is a
+  // multiple of the power-of-two tile edge (or of the stage depth) and d is
+  // below it.
+  uint64_t a_base = 0, b_base = 0, a_span, b_span;  // in elements
+  if constexpr (!WIDE) {
+    if constexpr (AM == 0) {
+      a_span = uint64_t(M) * uint64_t(K);
+      b_span = uint64_t(K) * uint64_t(N);
+    } else {
+      a_span = map_offset(a_rows, uint64_t(M - 1)) + map_offset(a_cols, uint64_t(K - 1)) + 1;
+      b_span = map_offset(b_rows, uint64_t(K - 1)) + map_offset(b_cols, uint64_t(N - 1)) + 1;
+    }
+  } else if constexpr (AM == 0) {
     a_base = uint64_t(m0) * uint64_t(K);
     b_base = uint64_t(n0);
     a_span = uint64_t(kBM - 1) * uint64_t(K) + kBBK;
@@ -689,13 +700,13 @@ __device__ __forceinline__ void split_big_tile(
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + NTH * h;
       const int row = AM == 1 ? (f >> 2) : 2 * (f >> 3);
-      a_off[h] = map_offset(a_rows, uint64_t(row));
+      a_off[h] = map_offset(a_rows, uint64_t(WIDE ? row : m0 + row));
     }
     a_coff = map_offset(a_cols, AM == 1 ? uint64_t(2 * (tid & 3)) : uint64_t(tid & 7));
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
-      b_off[h] = map_offset(b_cols, uint64_t(2 * (g >> 3)));
+      b_off[h] = map_offset(b_cols, uint64_t(WIDE ? 2 * (g >> 3) : n0 + 2 * (g >> 3)));
     }
     b_roff = map_offset(b_rows, uint64_t(tid & 7));
   }
@@ -714,8 +725,15 @@ __device__ __forceinline__ void split_big_tile(
   // tile's range gets an offset past the end of the buffer and reads as 0,
   // so there is no branch and no select near the loads (the waits for a
   // stage's operands then never include the next stage's loads).  The host
-  // checks that both spans fit (big_operands_fit, big_mapped_spans_fit).
-  const int a_rec = int(a_span * sizeof(float2)), b_rec = int(b_span * sizeof(float2));
+  // checks that both spans fit (big_operands_whole, big_mapped_whole; WIDE:
+  // big_operands_fit, big_mapped_spans_fit).
+  // (uniform values: made scalar so that the resources stay in SGPRs)
+  const int a_rec = __builtin_amdgcn_readfirstlane(int(a_span * sizeof(float2)));
+  const int b_rec = __builtin_amdgcn_readfirstlane(int(b_span * sizeof(float2)));
+  const __amdgpu_buffer_rsrc_t rsrc_a_whole =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(A), 0, a_rec, 0x00020000);
+  const __amdgpu_buffer_rsrc_t rsrc_b_whole =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(B), 0, b_rec, 0x00020000);
   constexpr unsigned kOutside = 0xfffffff0u;
   auto load16 = [](__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
     const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, int(byte_off), 0, 0);
@@ -724,8 +742,14 @@ __device__ __forceinline__ void split_big_tile(
   };
   auto load_tile = [&](int kt, float4* ra, float4* rb) {
     const int k0 = kt * kBBK;
-    uint64_t a_stage, b_stage;  // the stage's first k, in elements from the block bases
-    if constexpr (AM != 0) {
+    uint64_t a_stage = 0, b_stage = 0;  // WIDE: the stage's first k, from the block bases
+    unsigned a_kbase = 0, b_kbase = 0;  // otherwise: its offsets from the operand bases
+    if constexpr (!WIDE) {
+      if constexpr (AM != 0) {
+        a_kbase = unsigned(ktab[2 * kt]);
+        b_kbase = unsigned(ktab[2 * kt + 1]);
+      }
+    } else if constexpr (AM != 0) {
       // uniform values read from LDS: made scalar so the resources stay in SGPRs
       const uint64_t ta = ktab[2 * kt], tb = ktab[2 * kt + 1];
       a_stage = uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(ta))))) |
@@ -736,10 +760,16 @@ __device__ __forceinline__ void split_big_tile(
       a_stage = uint64_t(k0);
       b_stage = uint64_t(k0) * uint64_t(N);
     }
-    const __amdgpu_buffer_rsrc_t rsrc_a = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float2*>(A + a_base + a_stage), 0, a_rec, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rsrc_b = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float2*>(B + b_base + b_stage), 0, b_rec, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsrc_a =
+        WIDE ? __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(A + a_base + a_stage), 0,
+                                                 a_rec, 0x00020000)
+             : rsrc_a_whole;
+    const __amdgpu_buffer_rsrc_t rsrc_b =
+        WIDE ? __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(B + b_base + b_stage), 0,
+                                                 b_rec, 0x00020000)
+             : rsrc_b_whole;
+    // row and column origins of the offsets: the block's (WIDE) or 0
+    const int mo = WIDE ? m0 : 0, no = WIDE ? n0 : 0, ko = WIDE ? k0 : 0;
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + NTH * h;
@@ -748,12 +778,12 @@ __device__ __forceinline__ void split_big_tile(
       if constexpr (AM == 2) {
         gm = m0 + 2 * (f >> 3);
         gk = k0 + (f & 7);
-        off = unsigned(a_off[h]) + unsigned(a_coff);
+        off = unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
       } else {
         gm = m0 + (f >> 2);
         gk = k0 + 2 * (f & 3);
-        off = AM == 0 ? unsigned(gm - m0) * unsigned(K) + unsigned(gk - k0)
-                      : unsigned(a_off[h]) + unsigned(a_coff);
+        off = AM == 0 ? unsigned(gm - mo) * unsigned(K) + unsigned(gk - ko)
+                      : unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
       }
       ra[h] = load16(rsrc_a, (gm < M && gk < K) ? off * unsigned(sizeof(float2)) : kOutside);
     }
@@ -761,8 +791,8 @@ __device__ __forceinline__ void split_big_tile(
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
       const int gk = k0 + (g & 7), gn = n0 + 2 * (g >> 3);
-      const unsigned off = AM == 0 ? unsigned(gk - k0) * unsigned(N) + unsigned(gn - n0)
-                                   : unsigned(b_roff) + unsigned(b_off[h]);
+      const unsigned off = AM == 0 ? unsigned(gk - ko) * unsigned(N) + unsigned(gn - no)
+                                   : b_kbase + unsigned(b_roff) + unsigned(b_off[h]);
       const bool in = (kBT % NTH == 0 || g < kBT) && gk < K && gn < N;
       rb[h] = load16(rsrc_b, in ? off * unsigned(sizeof(float2)) : kOutside);
     }
@@ -946,13 +976,13 @@ __device__ __forceinline__ void split_big_tile(
   }
 }
 
-template <int MODE, int WM, int WN, int J, int AM, int I = 2>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2, bool WIDE = false>
 __global__ __launch_bounds__(64 * WM * WN) void cgemm_split_big_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
     int N, int K, AmaxRef amax, qfx_index_map a_rows,
     qfx_index_map a_cols, qfx_index_map b_rows, qfx_index_map b_cols,
     float* __restrict__ cmax) {
-  split_big_tile<MODE, WM, WN, J, AM, I>(
+  split_big_tile<MODE, WM, WN, J, AM, I, WIDE>(
       A, B, C, M, N, K, amax, a_rows, a_cols, b_rows, b_cols,
       cmax ? cmax + blockIdx.x + gridDim.x * blockIdx.y : nullptr, int(blockIdx.x) * 32 * I * WM,
       int(blockIdx.y) * 16 * J * WN);
@@ -1006,25 +1036,25 @@ size_t split_big_lds(uint64_t k) {
 }
 
 // Dynamic LDS above 64 KiB must be allowed per kernel once per process.
-template <int MODE, int WM, int WN, int J, int AM, int I = 2>
+template <int MODE, int WM, int WN, int J, int AM, int I = 2, bool WIDE = false>
 hipError_t split_big_prepare() {
   static hipError_t e = hipFuncSetAttribute(
-      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN, J, AM, I>),
+      reinterpret_cast<const void*>(&cgemm_split_big_kernel<MODE, WM, WN, J, AM, I, WIDE>),
       hipFuncAttributeMaxDynamicSharedMemorySize,
       int(AM != 0 ? kMaxLds : split_big_smem<MODE, WM, WN, J, I>()));
   return e;
 }
 
-template <int MODE, int WM, int WN, int J, int AM, int I = 2>
+template <int MODE, int WM, int WN, int J, int AM, bool WIDE, int I = 2>
 hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
                             uint64_t m, uint64_t n, uint64_t k, AmaxRef amax,
                             const qfx_index_map* maps, float* cmax) {
-  hipError_t e = split_big_prepare<MODE, WM, WN, J, AM, I>();
+  hipError_t e = split_big_prepare<MODE, WM, WN, J, AM, I, WIDE>();
   if (e != hipSuccess) return e;
   constexpr int kBM = 32 * I * WM, kBN = 16 * J * WN;
   dim3 g{unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN), 1};
   qfx_index_map none{};
-  cgemm_split_big_kernel<MODE, WM, WN, J, AM, I>
+  cgemm_split_big_kernel<MODE, WM, WN, J, AM, I, WIDE>
       <<<g, 64 * WM * WN, split_big_lds<MODE, WM, WN, J, AM, I>(k), stream>>>(
           A, B, C, int(m), int(n), int(k), amax, maps ? maps[0] : none, maps ? maps[1] : none,
           maps ? maps[2] : none, maps ? maps[3] : none, cmax);
@@ -1069,6 +1099,20 @@ bool big_mapped_spans_fit(const qfx_index_map& a_rows, const qfx_index_map& a_co
          m <= 0x7fffffffULL && n <= 0x7fffffffULL && k <= 0x7fffffffULL;
 }
 
+// Whether each operand, whole, fits the 32-bit offsets of one buffer
+// resource (the default large-tile addressing); otherwise the kernel bases
+// its resources per block and stage (WIDE).
+bool big_operands_whole(uint64_t m, uint64_t n, uint64_t k) {
+  return m * k * sizeof(float2) <= kSpanLimit && k * n * sizeof(float2) <= kSpanLimit;
+}
+bool big_mapped_whole(const qfx_index_map& a_rows, const qfx_index_map& a_cols,
+                      const qfx_index_map& b_rows, const qfx_index_map& b_cols, uint64_t m,
+                      uint64_t n, uint64_t k) {
+  const uint64_t a_span = map_offset_host(a_rows, m - 1) + map_offset_host(a_cols, k - 1) + 1;
+  const uint64_t b_span = map_offset_host(b_rows, k - 1) + map_offset_host(b_cols, n - 1) + 1;
+  return a_span * sizeof(float2) <= kSpanLimit && b_span * sizeof(float2) <= kSpanLimit;
+}
+
 BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
   auto blocks = [&](uint64_t bm, uint64_t bn) { return ((m + bm - 1) / bm) * ((n + bn - 1) / bn); };
   // 128 x 256 measured faster for tall operands (m >= 128 n), 256 x 128 otherwise
@@ -1082,19 +1126,19 @@ BigLayout choose_big_layout(int precision, uint64_t m, uint64_t n) {
   return kLayoutNone;
 }
 
-template <int MODE, int AM>
+template <int MODE, int AM, bool WIDE>
 hipError_t launch_split_big_layout(BigLayout lay, hipStream_t stream, const float2* A,
                                    const float2* B, float2* C, uint64_t m, uint64_t n,
                                    uint64_t k, AmaxRef amax,
                                    const qfx_index_map* maps, float* cmax) {
   if constexpr (MODE == 0) {
-    return launch_split_big<0, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
+    return launch_split_big<0, 4, 2, 4, AM, WIDE>(stream, A, B, C, m, n, k, amax, maps, cmax);
   } else {
     if (lay == kLayout2x4)
-      return launch_split_big<MODE, 2, 4, 4, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
+      return launch_split_big<MODE, 2, 4, 4, AM, WIDE>(stream, A, B, C, m, n, k, amax, maps, cmax);
     if (lay == kLayout8x1)
-      return launch_split_big<MODE, 8, 1, 2, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
-    return launch_split_big<MODE, 4, 2, 4, AM>(stream, A, B, C, m, n, k, amax, maps, cmax);
+      return launch_split_big<MODE, 8, 1, 2, AM, WIDE>(stream, A, B, C, m, n, k, amax, maps, cmax);
+    return launch_split_big<MODE, 4, 2, 4, AM, WIDE>(stream, A, B, C, m, n, k, amax, maps, cmax);
   }
 }
 
@@ -1106,16 +1150,29 @@ bool split_big_mapped_fits(int precision, BigLayout lay, uint64_t k) {
   return split_big_lds<2, 4, 2, 4, 1>(k) <= kMaxLds;
 }
 
-template <int AM>
-hipError_t launch_split_big_prec(int precision, BigLayout lay, hipStream_t stream,
+template <int AM, bool WIDE>
+hipError_t launch_split_big_mode(int precision, BigLayout lay, hipStream_t stream,
                                  const float2* A, const float2* B, float2* C, uint64_t m,
                                  uint64_t n, uint64_t k, AmaxRef amax,
                                  const qfx_index_map* maps, float* cmax) {
   if (precision == QFX_PREC_SPLIT_BF16X3)
-    return launch_split_big_layout<0, AM>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
+    return launch_split_big_layout<0, AM, WIDE>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
   if (precision == QFX_PREC_SPLIT_BF16)
-    return launch_split_big_layout<1, AM>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
-  return launch_split_big_layout<2, AM>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
+    return launch_split_big_layout<1, AM, WIDE>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
+  return launch_split_big_layout<2, AM, WIDE>(lay, stream, A, B, C, m, n, k, amax, maps, cmax);
+}
+
+// wide: at least one operand spans more than the 32-bit offsets of one
+// resource (per-stage resources then).
+template <int AM>
+hipError_t launch_split_big_prec(int precision, BigLayout lay, hipStream_t stream,
+                                 const float2* A, const float2* B, float2* C, uint64_t m,
+                                 uint64_t n, uint64_t k, AmaxRef amax,
+                                 const qfx_index_map* maps, float* cmax, bool wide) {
+  return wide ? launch_split_big_mode<AM, true>(precision, lay, stream, A, B, C, m, n, k, amax,
+                                                maps, cmax)
+              : launch_split_big_mode<AM, false>(precision, lay, stream, A, B, C, m, n, k, amax,
+                                                 maps, cmax);
 }
 
 // Blocks (one partial maximum each) of a large-tile launch.
@@ -2000,16 +2057,17 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
   if (layout != kLayoutNone) {
     if ((n + 31) / 32 > 65535) return qfx_set_error(QFX_ERR_UNSUPPORTED, "qfx_cgemm_c64: n too large");
     hipError_t te = hipSuccess;
+    const bool wide = !big_operands_whole(m, n, k);
     const BigLayout use = tuned_layout(
         ctx, precision, 0, m, n, k, layout,
         [&](BigLayout l) {
           return launch_split_big_prec<0>(precision, l, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                          nullptr, nullptr);
+                                          nullptr, nullptr, wide);
         },
         &te);
     QFX_HIP(te);
     QFX_HIP(launch_split_big_prec<0>(precision, use, ctx->stream, A2, B2, C2, m, n, k, amax,
-                                     nullptr, nullptr));
+                                     nullptr, nullptr, wide));
     return QFX_OK;
   }
   if (precision == QFX_PREC_SPLIT_FP16) {
@@ -2234,11 +2292,12 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
                ? io->c_max
                : nullptr;
   };
+  const bool wide = !big_mapped_whole(*a_rows, *a_cols, *b_rows, *b_cols, m, n, k);
   auto launch = [&](BigLayout l) {
     return am == 1 ? launch_split_big_prec<1>(precision, l, ctx->stream, A2, B2, C2, m, n, k,
-                                              amax, maps, cmax_for(l))
+                                              amax, maps, cmax_for(l), wide)
                    : launch_split_big_prec<2>(precision, l, ctx->stream, A2, B2, C2, m, n, k,
-                                              amax, maps, cmax_for(l));
+                                              amax, maps, cmax_for(l), wide);
   };
   hipError_t te = hipSuccess;
   const BigLayout use = tuned_layout(ctx, precision, am, m, n, k, layout, launch, &te);
@@ -2310,7 +2369,8 @@ extern "C" int qfx_cgemm_grouped_c64(qfx_context* ctx, int precision, int count,
                           d.k % 2 == 0 && d.n % 2 == 0 && d.A && d.B && d.C &&
                           reinterpret_cast<uintptr_t>(d.A) % 16 == 0 &&
                           reinterpret_cast<uintptr_t>(d.B) % 16 == 0 &&
-                          big_operands_fit(d.m, d.n, d.k) && d.m <= 0x7fffffffULL &&
+                          big_operands_fit(d.m, d.n, d.k) && big_operands_whole(d.m, d.n, d.k) &&
+                          d.m <= 0x7fffffffULL &&
                           d.n <= 0x7fffffffULL;
     if (eligible) {
       grouped.push_back(i);
