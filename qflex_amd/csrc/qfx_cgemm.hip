@@ -667,8 +667,9 @@ __device__ __forceinline__ void split_big_tile(
   // Normally (WIDE false) the resource covers the whole operand, made once.
   // WIDE: it is based at the block's first row (A) or column (B) and at the
   // stage's first k, so only one stage's footprint in the block, not the
-  // whole operand, has to fit the offsets (operands above 2 GiB; measured
-  // ~6% slower, so used only where the whole operand does not fit).  For a
+  // whole operand, has to fit the offsets (operands above 2 GiB; some scalar
+  // work per stage, within the noise of the whole-operand form in time, used
+  // only where the whole operand does not fit).  For a
   // digit map, offset(i0 + d) = offset(i0) + offset(d) when i0 is a
   // multiple of the power-of-two tile edge (or of the stage depth) and d is
   // below it.
