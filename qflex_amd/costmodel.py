@@ -58,12 +58,9 @@ def random_candidate(I, J, rng):
     q = lambda s: s[0] * J + s[1]
     sites = {(i, j) for i in range(I) for j in range(J)}
     rnd = lambda: bool(rng.integers(2))
-    u = rng.random()
-    if u < 0.2:
+    if rng.random() < 0.25:
         order = _sweep(sites, int(rng.integers(2)), rnd(), rnd(), rnd())
         return "".join(f"expand a {q(s)}\n" for s in order)
-    if u < 0.45 and I > 1 and J > 1:
-        return _corner_candidate(I, J, rng)
     axis = int(rng.integers(2))  # 0: split between rows, 1: between columns
     size = I if axis == 0 else J
     cut_at = int(rng.integers(1, size))
@@ -85,37 +82,6 @@ def random_candidate(I, J, rng):
             order = _sweep(part, int(rng.integers(2)), rnd(), rnd(), toward)
         lines.extend(f"expand {name} {q(s)}" for s in order)
     lines.append("merge a b")
-    return "\n".join(lines) + "\n"
-
-
-def _corner_candidate(I, J, rng):
-    """Four patches grown from the grid's corners toward a random interior
-    point (r, c), merged around the square in a random rotation and
-    direction, with up to two bond cuts on the boundaries between them (the
-    shape of the reference's hand-written orderings for square grids)."""
-    q = lambda s: s[0] * J + s[1]
-    rnd = lambda: bool(rng.integers(2))
-    r, c = int(rng.integers(1, I)), int(rng.integers(1, J))
-    quads = [  # in order around the square, with their outer corners
-        ({(i, j) for i in range(r) for j in range(c)}, False, False),
-        ({(i, j) for i in range(r) for j in range(c, J)}, False, True),
-        ({(i, j) for i in range(r, I) for j in range(c, J)}, True, True),
-        ({(i, j) for i in range(r, I) for j in range(c)}, True, False),
-    ]
-    start, step = int(rng.integers(4)), (1 if rnd() else -1)
-    order = [quads[(start + step * n) % 4] for n in range(4)]
-    boundary = [((i, c - 1), (i, c)) for i in range(I)] + [((r - 1, j), (r, j)) for j in range(J)]
-    lines = []
-    ncut = int(rng.choice([0, 1, 1, 2]))
-    for k in rng.choice(len(boundary), size=min(ncut, len(boundary)), replace=False):
-        a, b = boundary[int(k)]
-        lines.append(f"cut () {q(a)} {q(b)}")
-    names = ["a", "b", "c", "d"]
-    for name, (part, flip_rows, flip_cols) in zip(names, order):
-        # each quadrant swept from its outer corner inward
-        lines.extend(f"expand {name} {q(s)}"
-                     for s in _sweep(part, int(rng.integers(2)), rnd(), flip_rows, flip_cols))
-    lines += ["merge a b", "merge b c", "merge c d"]
     return "\n".join(lines) + "\n"
 
 

@@ -314,25 +314,6 @@ def test_costmodel_candidates_are_valid_orderings(I, J):
         assert all(lines.index(c) < first_expand for c in cuts)
 
 
-@pytest.mark.parametrize("I,J", [(2, 2), (3, 5), (6, 6), (8, 7)])
-def test_costmodel_corner_candidates(I, J):
-    # four corner patches merged around the square: a valid ordering with
-    # every site once, cuts only on bonds between different patches
-    from qflex_amd import circuits, costmodel
-    grid = circuits.grid_text(I, J).splitlines()
-    for seed in range(12):
-        order = costmodel._corner_candidate(I, J, np.random.default_rng(seed))
-        lines = order.splitlines()
-        qflex.validate_ordering(qflex.parse_ordering(lines, grid))
-        owner = {int(l.split()[2]): l.split()[1] for l in lines if l.startswith("expand")}
-        assert sorted(owner) == list(range(I * J)) and set(owner.values()) == set("abcd")
-        for l in lines:
-            if l.startswith("cut"):
-                a, b = int(l.split()[2]), int(l.split()[3])
-                assert owner[a] != owner[b] and abs(a - b) in (1, J)
-        assert lines[-3:] == ["merge a b", "merge b c", "merge c d"]
-
-
 def test_costmodel_features_from_a_plan():
     from qflex_amd import costmodel
     plan = {"steps": [{"m": 256, "n": 16, "k": 16, "multiplicity": 1, "out_rank": 3,
