@@ -36,6 +36,41 @@ struct BitPermParams {
 
 __device__ __forceinline__ uint32_t lds_slot(uint32_t s) { return s + (s >> 5); }
 
+// Offsets of tile element e: the input offset, output offset and LDS
+// position are OR-sums over e's set bits.  Elements e = t + e_j with t
+// (this thread's bits) and e_j (the unrolled iteration's bits, the same in
+// every lane) disjoint split each offset into a per-lane term computed once
+// and a per-iteration term that lives in scalar registers.  lds_slot is
+// additive over such a split: the low five bits of the two terms are disjoint,
+// so their sum carries nothing past bit 4.
+struct TileOffsets {
+  uint64_t in, out;
+  uint32_t slot;
+};
+
+template <int TBITS>
+__device__ __forceinline__ TileOffsets tile_offsets(const BitPermParams& p, uint32_t e, int k0) {
+  TileOffsets r{0, 0, 0};
+  for (int k = k0; k < TBITS; ++k)
+    if ((e >> k) & 1u) {
+      r.in |= uint64_t(1) << p.tin[k];
+      r.out |= uint64_t(1) << p.tout[k];
+      r.slot |= 1u << p.trank[k];
+    }
+  return r;
+}
+
+__device__ __forceinline__ void tile_base(const BitPermParams& p, uint64_t t, uint64_t& bin,
+                                          uint64_t& bout) {
+  bin = 0;
+  bout = 0;
+  for (int k = 0; k < p.nfree; ++k)
+    if ((t >> k) & 1u) {
+      bin |= uint64_t(1) << p.fin[k];
+      bout |= uint64_t(1) << p.fout[k];
+    }
+}
+
 template <int TBITS, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void permute_bits_kernel(const float2* __restrict__ src,
                                                              float2* __restrict__ dst,
@@ -45,55 +80,26 @@ __global__ __launch_bounds__(BLOCK) void permute_bits_kernel(const float2* __res
   static_assert(PER >= 1, "tile smaller than block");
   __shared__ float2 lds[TILE + TILE / 32];
 
+  // Tile-local element e = tid + j*BLOCK.
   const uint32_t tid = threadIdx.x;
-  // Offsets of this thread's elements inside a tile.  Tile-local element e
-  // = tid + j*BLOCK; offsets are linear in e's bits, so split lo/hi parts.
-  uint64_t in_lo = 0, out_lo = 0;
-  uint32_t slot_lo = 0;
-  for (int k = 0; k < TBITS; ++k) {
-    if ((tid >> k) & 1u) {
-      in_lo |= uint64_t(1) << p.tin[k];
-      slot_lo |= 1u << p.trank[k];
-      out_lo |= uint64_t(1) << p.tout[k];
-    }
-  }
-  uint64_t in_hi[PER], out_hi[PER];
-  uint32_t slot_hi[PER];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    uint32_t e = uint32_t(j) * BLOCK;
-    uint64_t a = 0, o = 0;
-    uint32_t s = 0;
-    for (int k = 0; k < TBITS; ++k) {
-      if ((e >> k) & 1u) {
-        a |= uint64_t(1) << p.tin[k];
-        s |= 1u << p.trank[k];
-        o |= uint64_t(1) << p.tout[k];
-      }
-    }
-    in_hi[j] = a | in_lo;
-    slot_hi[j] = lds_slot(s | slot_lo);
-    out_hi[j] = o | out_lo;
-  }
+  const TileOffsets lane = tile_offsets<TBITS>(p, tid, 0);
+  const uint32_t lane_slot = lds_slot(lane.slot);
 
   for (uint64_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
-    uint64_t bin = 0, bout = 0;
-    for (int k = 0; k < p.nfree; ++k) {
-      if ((t >> k) & 1u) {
-        bin |= uint64_t(1) << p.fin[k];
-        bout |= uint64_t(1) << p.fout[k];
-      }
-    }
+    uint64_t bin, bout;
+    tile_base(p, t, bin, bout);
     float2 v[PER];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) v[j] = src[bin + in_hi[j]];
+    for (int j = 0; j < PER; ++j)
+      v[j] = src[bin + (tile_offsets<TBITS>(p, uint32_t(j) * BLOCK, 0).in | lane.in)];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) lds[slot_hi[j]] = v[j];
+    for (int j = 0; j < PER; ++j)
+      lds[lds_slot(tile_offsets<TBITS>(p, uint32_t(j) * BLOCK, 0).slot) + lane_slot] = v[j];
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      uint32_t e = tid + uint32_t(j) * BLOCK;
-      dst[bout + out_hi[j]] = lds[lds_slot(e)];
+      const uint32_t ej = uint32_t(j) * BLOCK;
+      dst[bout + (tile_offsets<TBITS>(p, ej, 0).out | lane.out)] = lds[lds_slot(ej) + lds_slot(tid)];
     }
     __syncthreads();
   }
@@ -112,52 +118,36 @@ __global__ __launch_bounds__(BLOCK) void permute_pairs_kernel(const float2* __re
   static_assert(PER >= 1, "tile smaller than two elements per thread");
   __shared__ float2 lds[TILE + TILE / 32];
 
+  // Pair tid + j*BLOCK covers tile elements e = 2*(tid + j*BLOCK) and e + 1.
   const uint32_t tid = threadIdx.x;
-  // Pair e2 = tid + j*BLOCK covers tile elements 2*e2, 2*e2 + 1.
-  uint64_t in_off[PER], out_off[PER];
-  uint32_t slot0[PER], slot1[PER], rslot[PER];
-  const uint32_t bit0_rank = 1u << p.trank[0];
-#pragma unroll
-  for (int j = 0; j < PER; ++j) {
-    const uint32_t e = 2u * (tid + uint32_t(j) * BLOCK);
-    uint64_t a = 0, o = 0;
-    uint32_t s = 0;
-    for (int k = 1; k < TBITS; ++k) {
-      if ((e >> k) & 1u) {
-        a |= uint64_t(1) << p.tin[k];
-        s |= 1u << p.trank[k];
-        o |= uint64_t(1) << p.tout[k];
-      }
-    }
-    in_off[j] = a;
-    out_off[j] = o;
-    slot0[j] = lds_slot(s);
-    slot1[j] = lds_slot(s | bit0_rank);
-    rslot[j] = lds_slot(e);
-  }
+  const TileOffsets lane = tile_offsets<TBITS>(p, 2u * tid, 1);
+  const uint32_t slot0 = lds_slot(lane.slot), slot1 = lds_slot(lane.slot | (1u << p.trank[0]));
+  const uint32_t rslot = lds_slot(2u * tid);
 
   for (uint64_t t = blockIdx.x; t < p.ntiles; t += gridDim.x) {
-    uint64_t bin = 0, bout = 0;
-    for (int k = 0; k < p.nfree; ++k) {
-      if ((t >> k) & 1u) {
-        bin |= uint64_t(1) << p.fin[k];
-        bout |= uint64_t(1) << p.fout[k];
-      }
-    }
+    uint64_t bin, bout;
+    tile_base(p, t, bin, bout);
     float4 v[PER];
 #pragma unroll
-    for (int j = 0; j < PER; ++j) v[j] = *reinterpret_cast<const float4*>(src + bin + in_off[j]);
+    for (int j = 0; j < PER; ++j) {
+      const uint64_t off = tile_offsets<TBITS>(p, 2u * uint32_t(j) * BLOCK, 1).in | lane.in;
+      v[j] = *reinterpret_cast<const float4*>(src + bin + off);
+    }
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      lds[slot0[j]] = make_float2(v[j].x, v[j].y);
-      lds[slot1[j]] = make_float2(v[j].z, v[j].w);
+      const uint32_t sj = lds_slot(tile_offsets<TBITS>(p, 2u * uint32_t(j) * BLOCK, 1).slot);
+      lds[sj + slot0] = make_float2(v[j].x, v[j].y);
+      lds[sj + slot1] = make_float2(v[j].z, v[j].w);
     }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-      const float2 x = lds[rslot[j]];
-      const float2 y = lds[rslot[j] + 1];
-      *reinterpret_cast<float4*>(dst + bout + out_off[j]) = make_float4(x.x, x.y, y.x, y.y);
+      const uint32_t ej = 2u * uint32_t(j) * BLOCK;
+      const uint32_t r = lds_slot(ej) + rslot;
+      const float2 x = lds[r];
+      const float2 y = lds[r + 1];
+      const uint64_t off = tile_offsets<TBITS>(p, ej, 1).out | lane.out;
+      *reinterpret_cast<float4*>(dst + bout + off) = make_float4(x.x, x.y, y.x, y.y);
     }
     __syncthreads();
   }
