@@ -956,7 +956,10 @@ __device__ __forceinline__ void split_big_tile(
         const int cn = n0 + wn * 16 * J + j * 16 + (col >> 1);
         if (row < M && cn < N) {
           const float v = acc[i][j][r] * unscale;
-          Cf[(uint64_t(row) * N + cn) * 2 + part] = v;
+          // Streaming stores: C of these products is 128 MiB to 1 GiB, and
+          // its lines left dirty in the last-level cache slow the kernels
+          // that stream through it next (0.9% of the 7 x 7 depth-32 job).
+          __builtin_nontemporal_store(v, &Cf[(uint64_t(row) * N + cn) * 2 + part]);
           cm = fmaxf(cm, fabsf(v));
         }
       }
