@@ -1765,6 +1765,81 @@ __global__ __launch_bounds__(256) void cgemm_rows_kernel(
   rows_tile<MAPPED, AROW>(A, B, C, M, N, K, a_rows, a_cols, b_rows, b_cols, cmax, blockIdx.x);
 }
 
+// The row kernel for many 256-row tiles with A read one value per lane and
+// k (rows_tile without AROW): each block walks the tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ... and loads the next tile's row of A into
+// registers while it computes the current one.  With one tile per block the
+// blocks on a CU load together and then compute together (16 x 16 complex
+// fmaf chains per row, 27 us of vector work per 2^20 rows), so the loads and
+// the arithmetic do not overlap.  Same fmaf chains as rows_tile
+// (bit-identical); per-tile maxima in cmax[tile].
+constexpr unsigned kRowsStreamBlocks = 1024;  // 4 blocks on each of 256 CUs
+
+template <bool MAPPED>
+__global__ __launch_bounds__(256, 4) void cgemm_rows_stream_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
+    uint64_t M, int N, int K, qfx_index_map a_rows, qfx_index_map a_cols, qfx_index_map b_rows,
+    qfx_index_map b_cols, float* __restrict__ cmax, uint64_t ntiles) {
+  constexpr int kCPitch = 17;
+  __shared__ float2 Bs[kSmallK][16];
+  __shared__ uint64_t acol[kSmallK];
+  __shared__ float2 X[256 * kCPitch];
+  {
+    const int r = threadIdx.x >> 4, c = threadIdx.x & 15;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < K && c < N)
+      v = MAPPED ? B[map_offset(b_rows, r) + map_offset(b_cols, c)] : B[r * N + c];
+    Bs[r][c] = v;
+    if (threadIdx.x < kSmallK)
+      acol[threadIdx.x] = MAPPED && int(threadIdx.x) < K ? map_offset(a_cols, threadIdx.x)
+                                                          : threadIdx.x;
+  }
+  __syncthreads();
+  auto load_row = [&](uint64_t tile, float2* a) {
+    const uint64_t row = tile * 256 + threadIdx.x;
+#pragma unroll
+    for (int kk = 0; kk < kSmallK; ++kk) a[kk] = make_float2(0.f, 0.f);
+    if (row < M) {
+      const uint64_t abase = MAPPED ? map_offset(a_rows, row) : row * uint64_t(K);
+#pragma unroll
+      for (int kk = 0; kk < kSmallK; ++kk)
+        if (kk < K) a[kk] = A[abase + acol[kk]];
+    }
+  };
+  float2 a_cur[kSmallK], a_next[kSmallK];
+  load_row(blockIdx.x, a_cur);
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    if (t + gridDim.x < ntiles) load_row(t + gridDim.x, a_next);
+    const uint64_t m0 = t * 256, row = m0 + threadIdx.x;
+    float2 acc[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) acc[j] = make_float2(0.f, 0.f);
+    float cm = 0.f;
+    if (row < M) {
+#pragma unroll
+      for (int kk = 0; kk < kSmallK; ++kk)
+        if (kk < K) {
+#pragma unroll
+          for (int j = 0; j < 16; ++j) acc[j] = cfma(a_cur[kk], Bs[kk][j], acc[j]);
+        }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        X[threadIdx.x * kCPitch + j] = acc[j];
+        if (j < N) cm = fmaxf(cm, fmaxf(fabsf(acc[j].x), fabsf(acc[j].y)));
+      }
+    }
+    __syncthreads();
+    const uint64_t rows = M - m0 < 256 ? M - m0 : 256;
+    const int total = int(rows) * N;
+    float2* out = C + m0 * uint64_t(N);
+    for (int e = threadIdx.x; e < total; e += 256) out[e] = X[(e / N) * kCPitch + e % N];
+    if (cmax) store_block_max(cm, cmax + t);
+    __syncthreads();  // X and the maximum's scratch are rewritten by the next tile
+#pragma unroll
+    for (int kk = 0; kk < kSmallK; ++kk) a_cur[kk] = a_next[kk];
+  }
+}
+
 // One problem of a grouped launch of the row kernel (row-major operands).
 struct RowsProb {
   const float2* A;
@@ -1971,6 +2046,9 @@ extern "C" int qfx_cgemm_c64(qfx_context* ctx, int precision, uint64_t m, uint64
       if (k % 2 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0)
         cgemm_rows_kernel<false, true><<<blocks, 256, 0, ctx->stream>>>(
             A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr);
+      else if (blocks >= 2 * kRowsStreamBlocks)
+        cgemm_rows_stream_kernel<false><<<kRowsStreamBlocks, 256, 0, ctx->stream>>>(
+            A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr, blocks);
       else
         cgemm_rows_kernel<false, false><<<blocks, 256, 0, ctx->stream>>>(
             A2, B2, C2, m, int(n), int(k), {}, {}, {}, {}, nullptr);
@@ -2220,6 +2298,10 @@ extern "C" int qfx_cgemm_mapped_ex_c64(qfx_context* ctx, int precision, const qf
           reinterpret_cast<uintptr_t>(A) % 16 == 0)
         cgemm_rows_kernel<true, true><<<unsigned(blocks), 256, 0, ctx->stream>>>(
             A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks));
+      else if (blocks >= 2 * kRowsStreamBlocks)
+        cgemm_rows_stream_kernel<true><<<kRowsStreamBlocks, 256, 0, ctx->stream>>>(
+            A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks),
+            blocks);
       else
         cgemm_rows_kernel<true, false><<<unsigned(blocks), 256, 0, ctx->stream>>>(
             A2, B2, C2, m, int(n), int(k), *a_rows, *a_cols, *b_rows, *b_cols, cmax_for(blocks));

@@ -138,6 +138,8 @@ GEMM_SHAPES = [
     (4096, 4096, 64), (8, 8, 1 << 20), (3, 5, 5000), (64, 64, 1024), (1024, 1024, 40),
     # small common dimension (vector-ALU tiles of 256x16 and 64x64)
     (4096, 512, 16), (1000, 20, 16), (300, 7, 3), (70000, 16, 16), (257, 65, 9),
+    # one thread per row, more than 2048 tiles of 256 rows: each block walks several tiles
+    (524365, 16, 15),
     # at most 8 x 8 outputs with long k (split-K with register accumulators)
     (1, 8, 4096), (7, 3, 1500), (8, 1, 2048),
     # small outputs, k a multiple of 16: split-K over the vector-ALU tiles
@@ -181,13 +183,16 @@ def test_cgemm_reference_known_answer(ctx):
         assert np.array_equal(got.ravel(), np.array([20, 14, 100, 78], dtype=np.complex64))
 
 
-@pytest.mark.parametrize("m,n,k", [(70000, 16, 16), (3000, 7, 11), (1000, 100, 16), (5000, 9, 6)])
+@pytest.mark.parametrize("m,n,k", [(70000, 16, 16), (3000, 7, 11), (1000, 100, 16), (5000, 9, 6),
+                                   (524365, 16, 15), (530000, 3, 7)])
 def test_small_k_kernels_bit_identical_to_defined_order(ctx, m, n, k):
     """k <= 16 runs on the vector-ALU kernels (one thread per row for n <= 16,
     tiles otherwise); appending a zero column to A and a zero row to B makes
-    the FP32_EXACT product go to the scalar kernel (one fmaf chain per output
-    in increasing k): the extra fmaf adds exact zeros, so the results must
-    agree bit for bit."""
+    the FP32_EXACT product go to the scalar kernel for k = 16 (one fmaf chain
+    per output in increasing k), and for smaller k to the same kernels with
+    one more k (for odd k and more than 2048 tiles of 256 rows: the
+    tile-walking row kernel against the one reading whole rows of A): the
+    extra fmaf adds exact zeros, so the results must agree bit for bit."""
     rng = np.random.default_rng(m + n + k)
     a = rand_c64(rng, (m, k))
     b = rand_c64(rng, (k, n))

@@ -173,27 +173,35 @@ def test_scale_chain_matches_recomputed_scales(ctx):
         buf.free()
 
 
-@pytest.mark.parametrize("k1,n1", [(16, 16), (16, 256)])
-def test_scale_chain_from_small_k_products(ctx, k1, n1):
+@pytest.mark.parametrize("k1,n1,a_along_m", [(16, 16, False), (16, 256, False), (16, 16, True)])
+def test_scale_chain_from_small_k_products(ctx, k1, n1, a_along_m):
     # The vector-ALU products (k <= 16: one thread per row for n <= 16, tiles
     # otherwise) also write per-block maxima of |C| for a split-FP16
     # consumer; the next product reads C with them: same scale, same bits.
-    rng = np.random.default_rng(13 + n1)
+    # a_along_m: A stored k-major, read one value per lane and k by the
+    # row kernel that walks several 256-row tiles per block (2048 here).
+    rng = np.random.default_rng(13 + n1 + a_along_m)
     m, n = 524288 if n1 == 16 else 32768, 128  # the second product needs >= 128 large tiles
     a = (rng.standard_normal((m, k1)) + 1j * rng.standard_normal((m, k1))).astype(np.complex64)
     b1 = (rng.standard_normal((k1, n1)) + 1j * rng.standard_normal((k1, n1))).astype(np.complex64)
     b2 = (rng.standard_normal((n1, n)) + 1j * rng.standard_normal((n1, n))).astype(np.complex64)
-    A, B1, B2 = ctx.upload(a), ctx.upload(b1), ctx.upload(b2)
+    A = ctx.upload(np.ascontiguousarray(a.T) if a_along_m else a)
+    B1, B2 = ctx.upload(b1), ctx.upload(b2)
     C1, C2, C2ref = ctx.alloc(m * n1 * 8), ctx.alloc(m * n * 8), ctx.alloc(m * n * 8)
     cap = 8192
     CM = ctx.alloc(4 * cap)
     nat = lambda r, c: (capi.index_map([r, c], [0]), capi.index_map([r, c], [1]))
+    a_maps = ((capi.index_map([k1, m], [1]), capi.index_map([k1, m], [0])) if a_along_m
+              else nat(m, k1))
     io1 = capi.ScaleIO(None, 0, None, 0, CM.ptr, cap, 0)
-    assert ctx.cgemm_mapped_ex(capi.SPLIT_FP16, A, *nat(m, k1), B1, *nat(k1, n1), C1, io1)
+    assert ctx.cgemm_mapped_ex(capi.SPLIT_FP16, A, *a_maps, B1, *nat(k1, n1), C1, io1)
     assert 0 < io1.c_max_n <= cap
     cmax = ctx.download(CM, np.float32, (io1.c_max_n,))
     c1 = ctx.download(C1, np.complex64, (m, n1))
     assert cmax.max() == max(np.abs(c1.real).max(), np.abs(c1.imag).max())
+    if n1 == 16:  # one maximum per 256 rows of C
+        per = np.maximum(np.abs(c1.real), np.abs(c1.imag)).reshape(-1, 256 * n1).max(axis=1)
+        assert io1.c_max_n == m // 256 and np.array_equal(cmax, per)
     # C1 = A B1 exactly as the defined-order kernels compute it
     ref1 = a.astype(np.complex128) @ b1.astype(np.complex128)
     assert np.linalg.norm(c1 - ref1) / np.linalg.norm(ref1) < 2e-6
@@ -220,6 +228,8 @@ ROWS_CASES = [
     ([16, 16, 16, 16], [16], [16], "common", "right"),   # 2^16 x 16 x 16, A along k
     ([16, 16, 8], [2, 8], [8], "common", "common"),      # k in two digits, B along k
     ([16, 16, 16, 4], [16], [16], "left", "right"),      # A along m
+    # 2^20 x 16 x 16, A along m: 4096 tiles of 256 rows, each block walks four
+    ([16, 16, 16, 16, 16], [16], [16], "left", "right"),
 ]
 
 
