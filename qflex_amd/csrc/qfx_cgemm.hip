@@ -630,12 +630,22 @@ __device__ __forceinline__ floatx16 mfma16(typename SplitTraits<MODE>::V a,
 // One output tile (rows m0.., complex columns n0..) of the product; the
 // block's 512 threads all call it.  cmax_slot: where the tile's max |C|
 // goes (or null).
-template <int MODE, int WM, int WN, int J, int AM, int I = 2, bool WIDE = false>
+// CHAIN: the block computes the tiles L0, L0 + Lstride, ... below Lend
+// (tile L at row tile L / gy, column tile L % gy) one after the other; the
+// last two stages of a tile load the first two stages of the next one, and
+// its last stage splits the next tile's first stage into LDS, so a tile's
+// prologue overlaps the previous tile's last stages and C write-back.
+// cmax_slot then points at the per-tile maxima (slot L).  Needs an even
+// number of k-stages and whole-operand addressing (not WIDE).
+template <int MODE, int WM, int WN, int J, int AM, int I = 2, bool WIDE = false,
+          bool CHAIN = false>
 __device__ __forceinline__ void split_big_tile(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
     int N, int K, AmaxRef amax, const qfx_index_map& a_rows, const qfx_index_map& a_cols,
     const qfx_index_map& b_rows, const qfx_index_map& b_cols, float* __restrict__ cmax_slot,
-    const int m0, const int n0) {
+    const int m0_first, const int n0_first, const unsigned L0 = 0, const unsigned Lstride = 0,
+    const unsigned Lend = 0, const unsigned gy = 1) {
+  static_assert(!(CHAIN && WIDE), "chained tiles use whole-operand addressing");
   constexpr int NTH = 64 * WM * WN;  // threads of the block
   static_assert(NTH == 512 || NTH == 256, "8 or 4 waves");
   typedef typename SplitTraits<MODE>::T T;
@@ -657,6 +667,7 @@ __device__ __forceinline__ void split_big_tile(
   const int wm = wave / WN;  // 32 I rows each
   const int wn = wave % WN;  // 16*J complex columns each
   float sa = 1.f, sb = 1.f;
+  int m0 = m0_first, n0 = n0_first;  // the tile being computed
 
   // global -> register assignment (f = tid + 512*h, g = tid + 512*h)
   // A, AM 0/1: row = f >> 2, q = f & 3 (complex k = 2q, 2q+1 of the stage)
@@ -696,21 +707,46 @@ __device__ __forceinline__ void split_big_tile(
              map_offset(b_rows, uint64_t(min(kBBK, K) - 1)) + 1;
   }
   uint64_t a_off[kAH], a_coff = 0, b_off[kBH], b_roff = 0;
-  if constexpr (AM != 0) {
+  // the next tile of a chain: its origin and its lanes' row / column offsets
+  // (32 bits: chained operands fit the whole-operand resources)
+  unsigned Lc = L0;
+  bool has_next = false;
+  int m0n = 0, n0n = 0;
+  unsigned a_offn[kAH], b_offn[kBH];
+  auto tile_rows = [&](int tm0, auto* aoff) {
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
       const int f = tid + NTH * h;
       const int row = AM == 1 ? (f >> 2) : 2 * (f >> 3);
-      a_off[h] = map_offset(a_rows, uint64_t(WIDE ? row : m0 + row));
+      aoff[h] = map_offset(a_rows, uint64_t(WIDE ? row : tm0 + row));
     }
-    a_coff = map_offset(a_cols, AM == 1 ? uint64_t(2 * (tid & 3)) : uint64_t(tid & 7));
+  };
+  auto tile_cols = [&](int tn0, auto* boff) {
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
-      b_off[h] = map_offset(b_cols, uint64_t(WIDE ? 2 * (g >> 3) : n0 + 2 * (g >> 3)));
+      boff[h] = map_offset(b_cols, uint64_t(WIDE ? 2 * (g >> 3) : tn0 + 2 * (g >> 3)));
     }
+  };
+  auto set_next = [&]() {  // the chain's tile after Lc, if any
+    has_next = Lc + Lstride < Lend;
+    if (has_next) {
+      const unsigned L = Lc + Lstride;
+      m0n = int(L / gy) * kBM;
+      n0n = int(L % gy) * kBN;
+      if constexpr (AM != 0) {
+        tile_rows(m0n, a_offn);
+        tile_cols(n0n, b_offn);
+      }
+    }
+  };
+  if constexpr (AM != 0) {
+    tile_rows(m0, a_off);
+    a_coff = map_offset(a_cols, AM == 1 ? uint64_t(2 * (tid & 3)) : uint64_t(tid & 7));
+    tile_cols(n0, b_off);
     b_roff = map_offset(b_rows, uint64_t(tid & 7));
   }
+  if constexpr (CHAIN) set_next();
   // Per-stage k offsets of A and B (uniform), behind the two LDS buffers,
   // so that the pipelined stage has no digit arithmetic.
   uint64_t* ktab = reinterpret_cast<uint64_t*>(smem_raw + 2 * kBuf * sizeof(T));
@@ -741,8 +777,9 @@ __device__ __forceinline__ void split_big_tile(
     return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z),
                        __uint_as_float(v.w));
   };
-  auto load_tile = [&](int kt, float4* ra, float4* rb) {
+  auto load_tile = [&](int kt, float4* ra, float4* rb, bool nx = false) {
     const int k0 = kt * kBBK;
+    const int tm0 = CHAIN && nx ? m0n : m0, tn0 = CHAIN && nx ? n0n : n0;
     uint64_t a_stage = 0, b_stage = 0;  // WIDE: the stage's first k, from the block bases
     unsigned a_kbase = 0, b_kbase = 0;  // otherwise: its offsets from the operand bases
     if constexpr (!WIDE) {
@@ -776,24 +813,26 @@ __device__ __forceinline__ void split_big_tile(
       const int f = tid + NTH * h;
       int gm, gk;
       unsigned off;
+      const unsigned arow = CHAIN && nx ? a_offn[h] : unsigned(a_off[h]);
       if constexpr (AM == 2) {
-        gm = m0 + 2 * (f >> 3);
+        gm = tm0 + 2 * (f >> 3);
         gk = k0 + (f & 7);
-        off = unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
+        off = arow + a_kbase + unsigned(a_coff);
       } else {
-        gm = m0 + (f >> 2);
+        gm = tm0 + (f >> 2);
         gk = k0 + 2 * (f & 3);
         off = AM == 0 ? unsigned(gm - mo) * unsigned(K) + unsigned(gk - ko)
-                      : unsigned(a_off[h]) + a_kbase + unsigned(a_coff);
+                      : arow + a_kbase + unsigned(a_coff);
       }
       ra[h] = load16(rsrc_a, (gm < M && gk < K) ? off * unsigned(sizeof(float2)) : kOutside);
     }
 #pragma unroll
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
-      const int gk = k0 + (g & 7), gn = n0 + 2 * (g >> 3);
+      const int gk = k0 + (g & 7), gn = tn0 + 2 * (g >> 3);
+      const unsigned bcol = CHAIN && nx ? b_offn[h] : unsigned(b_off[h]);
       const unsigned off = AM == 0 ? unsigned(gk - ko) * unsigned(N) + unsigned(gn - no)
-                                   : b_kbase + unsigned(b_roff) + unsigned(b_off[h]);
+                                   : b_kbase + unsigned(b_roff) + bcol;
       const bool in = (kBT % NTH == 0 || g < kBT) && gk < K && gn < N;
       rb[h] = load16(rsrc_b, in ? off * unsigned(sizeof(float2)) : kOutside);
     }
@@ -929,13 +968,21 @@ __device__ __forceinline__ void split_big_tile(
   // follows it.
   auto stage = [&](auto si, int s) {
     constexpr int SI = decltype(si)::value;
-    load_tile(min(s + 2, nkt - 1), ra[SI], rb[SI]);
+    if constexpr (CHAIN) {
+      // the tile's last two stages load the next tile's first two
+      const bool nx = has_next && s + 2 >= nkt;
+      load_tile(nx ? s + 2 - nkt : min(s + 2, nkt - 1), ra[SI], rb[SI], nx);
+    } else {
+      load_tile(min(s + 2, nkt - 1), ra[SI], rb[SI]);
+    }
     __builtin_amdgcn_sched_barrier(0);
     compute(SI);
     store_tile(SI ^ 1, ra[SI ^ 1], rb[SI ^ 1]);
     schedule();
     __syncthreads();
   };
+  const float unscale = 1.f / (sa * sb);
+  for (;;) {  // the tiles of a chain (one tile otherwise)
   int s0 = 0;
   for (; s0 + 2 <= nkt; s0 += 2) {
     stage(std::integral_constant<int, 0>{}, s0);
@@ -943,7 +990,6 @@ __device__ __forceinline__ void split_big_tile(
   }
   if (s0 < nkt) compute(0);  // stage nkt - 1
 
-  const float unscale = 1.f / (sa * sb);
   float* Cf = reinterpret_cast<float*>(C);
   float cm = 0.f;  // max |re|, |im| of the values this thread stores
 #pragma unroll
@@ -964,10 +1010,13 @@ __device__ __forceinline__ void split_big_tile(
         }
       }
   if (cmax_slot) {
-    // one partial maximum per block, for the split of the GEMM that reads C
+    // one partial maximum per tile, for the split of the GEMM that reads C
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) cm = fmaxf(cm, __shfl_xor(cm, off));
-    float* red = reinterpret_cast<float*>(smem_raw);
+    // (a chain's next tile already has its first stage in buffer 0: the
+    // maxima then go behind the stage buffers and the k table)
+    float* red = reinterpret_cast<float*>(
+        CHAIN ? smem_raw + 2 * kBuf * sizeof(T) + (AM != 0 ? 16 * nkt : 0) : smem_raw);
     __syncthreads();  // the last stage's fragment reads are done
     if (lane == 0) red[wave] = cm;
     __syncthreads();
@@ -975,8 +1024,30 @@ __device__ __forceinline__ void split_big_tile(
       float b = red[0];
 #pragma unroll
       for (int w = 1; w < NTH / 64; ++w) b = fmaxf(b, red[w]);
-      *cmax_slot = b;
+      cmax_slot[CHAIN ? Lc : 0] = b;
     }
+  }
+  if constexpr (!CHAIN) {
+    break;
+  } else {
+    if (!has_next) break;
+    Lc += Lstride;
+    m0 = m0n;
+    n0 = n0n;
+    if constexpr (AM != 0) {
+#pragma unroll
+      for (int h = 0; h < kAH; ++h) a_off[h] = a_offn[h];
+#pragma unroll
+      for (int h = 0; h < kBH; ++h) b_off[h] = b_offn[h];
+    }
+    set_next();
+#pragma unroll
+    for (int i = 0; i < I; ++i)
+#pragma unroll
+      for (int j = 0; j < J; ++j)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+  }
   }
 }
 
@@ -994,6 +1065,21 @@ __global__ __launch_bounds__(64 * WM * WN) void cgemm_split_big_kernel(
   split_big_tile<MODE, WM, WN, J, AM, I, WIDE>(
       A, B, C, M, N, K, amax, a_rows, a_cols, b_rows, b_cols, cmax ? cmax + L : nullptr,
       int(tm) * 32 * I * WM, int(tn) * 16 * J * WN);
+}
+
+// Chained tiles (split_big_tile with CHAIN): one block per CU, block b
+// computing tiles b, b + gridDim.x, ... in the column-fastest order of
+// cgemm_split_big_kernel; cmax (if any) gets one maximum per tile.
+template <int MODE, int WM, int WN, int J, int AM, int I = 2>
+__global__ __launch_bounds__(64 * WM * WN) void cgemm_split_chain_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C, int M,
+    int N, int K, AmaxRef amax, qfx_index_map a_rows, qfx_index_map a_cols,
+    qfx_index_map b_rows, qfx_index_map b_cols, float* __restrict__ cmax, unsigned ntiles,
+    unsigned gy) {
+  const unsigned L = blockIdx.x;
+  split_big_tile<MODE, WM, WN, J, AM, I, false, true>(
+      A, B, C, M, N, K, amax, a_rows, a_cols, b_rows, b_cols, cmax, int(L / gy) * 32 * I * WM,
+      int(L % gy) * 16 * J * WN, L, gridDim.x, ntiles, gy);
 }
 
 // One problem of a grouped launch: row-major operands, 256 x 128 tiles.
@@ -1053,6 +1139,27 @@ hipError_t split_big_prepare() {
   return e;
 }
 
+// Compute units of the current device (the grid of a chained launch).
+unsigned device_cus() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cached[dev]) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      c = 0;
+    cached[dev] = c > 0 ? c : 256;
+  }
+  return unsigned(cached[dev]);
+}
+
+// Products whose tiles run as chains: whole-operand addressing, an even
+// number of k-stages, and at least two tiles for every compute unit.
+bool split_chain_applies(bool wide, uint64_t tiles, uint64_t k) {
+  const uint64_t nkt = (k + kBBK - 1) / kBBK;
+  return !wide && nkt % 2 == 0 && tiles >= 2 * uint64_t(device_cus()) && tiles <= 0xffffffffULL;
+}
+
 template <int MODE, int WM, int WN, int J, int AM, bool WIDE, int I = 2>
 hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B, float2* C,
                             uint64_t m, uint64_t n, uint64_t k, AmaxRef amax,
@@ -1062,6 +1169,24 @@ hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B
   constexpr int kBM = 32 * I * WM, kBN = 16 * J * WN;
   dim3 g{unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN), 1};
   qfx_index_map none{};
+  // (the 128 x 256 tile's digit-mapped forms would spill in a chain: the
+  // next tile's offsets do not fit beside its 250 VGPRs)
+  if constexpr (!WIDE && !(WM == 2 && WN == 4 && AM != 0)) {
+    const uint64_t tiles = uint64_t(g.x) * g.y;
+    // the per-tile maxima go behind the stage buffers and the k table
+    const size_t lds = split_big_lds<MODE, WM, WN, J, AM, I>(k) + 64;
+    if (lds <= kMaxLds && split_chain_applies(WIDE, tiles, k)) {
+      static hipError_t ce = hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&cgemm_split_chain_kernel<MODE, WM, WN, J, AM, I>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, int(kMaxLds));
+      if (ce != hipSuccess) return ce;
+      const unsigned grid = unsigned(std::min<uint64_t>(tiles, device_cus()));
+      cgemm_split_chain_kernel<MODE, WM, WN, J, AM, I><<<grid, 64 * WM * WN, lds, stream>>>(
+          A, B, C, int(m), int(n), int(k), amax, maps ? maps[0] : none, maps ? maps[1] : none,
+          maps ? maps[2] : none, maps ? maps[3] : none, cmax, unsigned(tiles), g.y);
+      return hipGetLastError();
+    }
+  }
   cgemm_split_big_kernel<MODE, WM, WN, J, AM, I, WIDE>
       <<<g, 64 * WM * WN, split_big_lds<MODE, WM, WN, J, AM, I>(k), stream>>>(
           A, B, C, int(m), int(n), int(k), amax, maps ? maps[0] : none, maps ? maps[1] : none,

@@ -122,3 +122,56 @@ def test_grouped_row_kernel_products_same_bits_as_alone(ctx, prec):
         ref = a.astype(np.complex128) @ b.astype(np.complex128)
         assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= TOL[prec], (m, n, k)
     free(probs)
+
+
+# Products with at least two 256 x 128 (or 128 x 256, 512 x 32) tiles per
+# compute unit and an even number of 8-deep k-stages run as chains of tiles
+# (each block computes several tiles, loading the next tile's first stages
+# during the current tile's last ones); the grouped launch runs one tile at a
+# time.  Same arithmetic per element: the same bits.
+CHAIN_SHAPES = [(65536, 256, 256), (131072, 512, 256), (70000, 300, 144), (262144, 32, 256),
+                (65536, 256, 64)]
+
+
+@pytest.mark.parametrize("m,n,k", CHAIN_SHAPES)
+def test_chained_tiles_bit_identical_to_single_tiles(ctx, m, n, k):
+    rng = np.random.default_rng(m + n + k)
+    a, b = rand_c(rng, (m, k)), rand_c(rng, (k, n))
+    A, B = ctx.upload(a), ctx.upload(b)
+    C, G = ctx.alloc(m * n * 8), ctx.alloc(m * n * 8)
+    ctx.cgemm(capi.SPLIT_FP16, m, n, k, A, B, C)
+    ctx.cgemm_grouped(capi.SPLIT_FP16, [(m, n, k, A, B, G)])
+    got = ctx.download(C, np.complex64, (m, n))
+    want = ctx.download(G, np.complex64, (m, n))
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (m, n, k)
+    rows = np.concatenate([rng.integers(0, m, 500), np.arange(m - 4, m)])
+    ref = a[rows].astype(np.complex128) @ b.astype(np.complex128)
+    assert np.linalg.norm(got[rows] - ref) / np.linalg.norm(ref) <= 5e-6
+    for buf in (A, B, C, G):
+        buf.free()
+
+
+@pytest.mark.parametrize("a_inner", ["left", "common"])
+def test_chained_mapped_tiles_bit_identical_to_single_tiles(ctx, a_inner):
+    """Operands read in their stored index orders (A along m or along k) by
+    chained tiles: the same bits as the grouped launch on the reordered
+    matrices."""
+    from test_mapped_gpu import make_case
+    a_dims, a_l, a_c, b_dims, b_c, b_r = make_case([16] * 4, [16, 16], [16, 16], a_inner, "right",
+                                                   seed=7)
+    rng = np.random.default_rng(7)
+    a, b = rand_c(rng, a_dims), rand_c(rng, b_dims)
+    m, k, n = 65536, 256, 256
+    am = np.ascontiguousarray(a.transpose(a_l + a_c).reshape(m, k))
+    bm = np.ascontiguousarray(b.transpose(b_c + b_r).reshape(k, n))
+    maps = (capi.index_map(a_dims, a_l), capi.index_map(a_dims, a_c),
+            capi.index_map(b_dims, b_c), capi.index_map(b_dims, b_r))
+    A, B, C = ctx.upload(a), ctx.upload(b), ctx.alloc(m * n * 8)
+    Am, Bm, G = ctx.upload(am), ctx.upload(bm), ctx.alloc(m * n * 8)
+    assert ctx.cgemm_mapped(capi.SPLIT_FP16, A, maps[0], maps[1], B, maps[2], maps[3], C)
+    ctx.cgemm_grouped(capi.SPLIT_FP16, [(m, n, k, Am, Bm, G)])
+    got = ctx.download(C, np.complex64, (m, n))
+    want = ctx.download(G, np.complex64, (m, n))
+    assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), a_inner
+    for buf in (A, B, C, Am, Bm, G):
+        buf.free()
