@@ -1169,9 +1169,10 @@ hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B
   constexpr int kBM = 32 * I * WM, kBN = 16 * J * WN;
   dim3 g{unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN), 1};
   qfx_index_map none{};
-  // (the 128 x 256 tile's digit-mapped forms would spill in a chain: the
-  // next tile's offsets do not fit beside its 250 VGPRs)
-  if constexpr (!WIDE && !(WM == 2 && WN == 4 && AM != 0)) {
+  // (the 128 x 256 tile's digit-mapped forms and the three-plane BF16 split
+  // would spill in a chain: the next tile's offsets do not fit beside their
+  // 250-256 VGPRs)
+  if constexpr (!WIDE && MODE != 0 && !(WM == 2 && WN == 4 && AM != 0)) {
     const uint64_t tiles = uint64_t(g.x) * g.y;
     // the per-tile maxima go behind the stage buffers and the k table
     const size_t lds = split_big_lds<MODE, WM, WN, J, AM, I>(k) + 64;
