@@ -712,7 +712,15 @@ __device__ __forceinline__ void split_big_tile(
   unsigned Lc = L0;
   bool has_next = false;
   int m0n = 0, n0n = 0;
-  unsigned a_offn[kAH], b_offn[kBH];
+  // The 128 x 256 tile's digit-mapped forms keep the next tile's offsets in
+  // LDS (each lane its own slots, behind the per-tile maxima): in registers
+  // they would spill.
+  constexpr bool kNextInLds = CHAIN && WM == 2 && WN == 4 && AM != 0;
+  unsigned a_offn[kNextInLds ? 1 : kAH], b_offn[kNextInLds ? 1 : kBH];
+  auto next_slots = [&]() {
+    return reinterpret_cast<unsigned*>(smem_raw + 2 * kBuf * sizeof(T) +
+                                       (AM != 0 ? 16 * ((K + kBBK - 1) / kBBK) : 0) + 64);
+  };
   auto tile_rows = [&](int tm0, auto* aoff) {
 #pragma unroll
     for (int h = 0; h < kAH; ++h) {
@@ -734,7 +742,16 @@ __device__ __forceinline__ void split_big_tile(
       const unsigned L = Lc + Lstride;
       m0n = int(L / gy) * kBM;
       n0n = int(L % gy) * kBN;
-      if constexpr (AM != 0) {
+      if constexpr (kNextInLds) {
+        uint64_t ta[kAH], tb[kBH];
+        tile_rows(m0n, ta);
+        tile_cols(n0n, tb);
+        unsigned* nl = next_slots();
+#pragma unroll
+        for (int h = 0; h < kAH; ++h) nl[h * NTH + tid] = unsigned(ta[h]);
+#pragma unroll
+        for (int h = 0; h < kBH; ++h) nl[(kAH + h) * NTH + tid] = unsigned(tb[h]);
+      } else if constexpr (AM != 0) {
         tile_rows(m0n, a_offn);
         tile_cols(n0n, b_offn);
       }
@@ -813,7 +830,11 @@ __device__ __forceinline__ void split_big_tile(
       const int f = tid + NTH * h;
       int gm, gk;
       unsigned off;
-      const unsigned arow = CHAIN && nx ? a_offn[h] : unsigned(a_off[h]);
+      unsigned arow;
+      if constexpr (kNextInLds)
+        arow = nx ? next_slots()[h * NTH + tid] : unsigned(a_off[h]);
+      else
+        arow = CHAIN && nx ? a_offn[h] : unsigned(a_off[h]);
       if constexpr (AM == 2) {
         gm = tm0 + 2 * (f >> 3);
         gk = k0 + (f & 7);
@@ -830,7 +851,11 @@ __device__ __forceinline__ void split_big_tile(
     for (int h = 0; h < kBH; ++h) {
       const int g = tid + NTH * h;
       const int gk = k0 + (g & 7), gn = tn0 + 2 * (g >> 3);
-      const unsigned bcol = CHAIN && nx ? b_offn[h] : unsigned(b_off[h]);
+      unsigned bcol;
+      if constexpr (kNextInLds)
+        bcol = nx ? next_slots()[(kAH + h) * NTH + tid] : unsigned(b_off[h]);
+      else
+        bcol = CHAIN && nx ? b_offn[h] : unsigned(b_off[h]);
       const unsigned off = AM == 0 ? unsigned(gk - ko) * unsigned(N) + unsigned(gn - no)
                                    : b_kbase + unsigned(b_roff) + bcol;
       const bool in = (kBT % NTH == 0 || g < kBT) && gk < K && gn < N;
@@ -1034,7 +1059,13 @@ __device__ __forceinline__ void split_big_tile(
     Lc += Lstride;
     m0 = m0n;
     n0 = n0n;
-    if constexpr (AM != 0) {
+    if constexpr (kNextInLds) {
+      const unsigned* nl = next_slots();
+#pragma unroll
+      for (int h = 0; h < kAH; ++h) a_off[h] = nl[h * NTH + tid];
+#pragma unroll
+      for (int h = 0; h < kBH; ++h) b_off[h] = nl[(kAH + h) * NTH + tid];
+    } else if constexpr (AM != 0) {
 #pragma unroll
       for (int h = 0; h < kAH; ++h) a_off[h] = a_offn[h];
 #pragma unroll
@@ -1169,13 +1200,17 @@ hipError_t launch_split_big(hipStream_t stream, const float2* A, const float2* B
   constexpr int kBM = 32 * I * WM, kBN = 16 * J * WN;
   dim3 g{unsigned((m + kBM - 1) / kBM), unsigned((n + kBN - 1) / kBN), 1};
   qfx_index_map none{};
-  // (the 128 x 256 tile's digit-mapped forms and the three-plane BF16 split
-  // would spill in a chain: the next tile's offsets do not fit beside their
-  // 250-256 VGPRs)
-  if constexpr (!WIDE && MODE != 0 && !(WM == 2 && WN == 4 && AM != 0)) {
+  // (the three-plane BF16 split would spill in a chain: the next tile's
+  // offsets do not fit beside its 256 VGPRs)
+  if constexpr (!WIDE && MODE != 0) {
     const uint64_t tiles = uint64_t(g.x) * g.y;
-    // the per-tile maxima go behind the stage buffers and the k table
-    const size_t lds = split_big_lds<MODE, WM, WN, J, AM, I>(k) + 64;
+    // the per-tile maxima go behind the stage buffers and the k table, and
+    // for the 128 x 256 tile's digit-mapped forms the next tile's lane
+    // offsets (4 bytes per lane and load) behind them
+    constexpr int kAH = 32 * I * WM * 4 / (64 * WM * WN);
+    constexpr int kBH = (4 * 16 * J * WN + 64 * WM * WN - 1) / (64 * WM * WN);
+    const size_t lds = split_big_lds<MODE, WM, WN, J, AM, I>(k) + 64 +
+                       (WM == 2 && WN == 4 && AM != 0 ? size_t(4) * (kAH + kBH) * 64 * WM * WN : 0);
     if (lds <= kMaxLds && split_chain_applies(WIDE, tiles, k)) {
       static hipError_t ce = hipFuncSetAttribute(
           reinterpret_cast<const void*>(&cgemm_split_chain_kernel<MODE, WM, WN, J, AM, I>),

@@ -152,16 +152,17 @@ def test_chained_tiles_bit_identical_to_single_tiles(ctx, m, n, k):
 
 
 @pytest.mark.parametrize("a_inner", ["left", "common"])
-def test_chained_mapped_tiles_bit_identical_to_single_tiles(ctx, a_inner):
+@pytest.mark.parametrize("left,right", [([16] * 4, [16, 16]),        # 256 x 128 tiles
+                                        ([2] + [16] * 4, [2, 16, 16])])  # 128 x 256 tiles
+def test_chained_mapped_tiles_bit_identical_to_single_tiles(ctx, a_inner, left, right):
     """Operands read in their stored index orders (A along m or along k) by
     chained tiles: the same bits as the grouped launch on the reordered
     matrices."""
     from test_mapped_gpu import make_case
-    a_dims, a_l, a_c, b_dims, b_c, b_r = make_case([16] * 4, [16, 16], [16, 16], a_inner, "right",
-                                                   seed=7)
+    a_dims, a_l, a_c, b_dims, b_c, b_r = make_case(left, [16, 16], right, a_inner, "right", seed=7)
     rng = np.random.default_rng(7)
     a, b = rand_c(rng, a_dims), rand_c(rng, b_dims)
-    m, k, n = 65536, 256, 256
+    m, k, n = int(np.prod(left)), 256, int(np.prod(right))
     am = np.ascontiguousarray(a.transpose(a_l + a_c).reshape(m, k))
     bm = np.ascontiguousarray(b.transpose(b_c + b_r).reshape(k, n))
     maps = (capi.index_map(a_dims, a_l), capi.index_map(a_dims, a_c),
