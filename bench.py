@@ -305,8 +305,8 @@ def library_fp16(m, n, k, warmup, steps):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cgemm_traffic_7x7_d32.json")
-MFMA_GEMM_KERNELS = ("cgemm_split_big_kernel", "cgemm_fp16x2_mfma_kernel", "cgemm_f32_mfma_kernel",
-                     "cgemm_bf16_mfma_kernel")
+MFMA_GEMM_KERNELS = ("cgemm_split_big_kernel", "cgemm_split_chain_kernel", "cgemm_fp16x2_mfma_kernel",
+                     "cgemm_f32_mfma_kernel", "cgemm_bf16_mfma_kernel")
 GEMM_HELPER_KERNELS = ("amax_partial_kernel",)
 
 
